@@ -1,0 +1,53 @@
+"""In-tree build of the native parts of s_spi_spring_amd.
+
+  lib/libspring_hip.so   HIP kernels + C ABI (include/spring_hip.h), gfx950 only
+
+Run ``python -m s_spi_spring_amd.build`` or call ``build_all()``. The build runs
+without a GPU (hipcc cross-compiles for gfx950).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "lib")
+INCLUDE = os.path.join(ROOT, "include")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# Packed-FP32 VALU forms are disabled for all device code.
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build step failed: " + " ".join(cmd))
+    return r
+
+
+def build_hip(verbose: bool = False) -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, "libspring_hip.so")
+    tmp = out + ".tmp"
+    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-I", INCLUDE, "-I", CSRC,
+           "-o", tmp, os.path.join(CSRC, "spr_backend.hip")]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
+def build_all(verbose: bool = False):
+    return [build_hip(verbose)]
+
+
+if __name__ == "__main__":
+    for p in build_all(verbose=True):
+        print("built", p)

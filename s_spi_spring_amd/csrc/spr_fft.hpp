@@ -1,0 +1,172 @@
+// Device-side FFT building blocks for gfx950 (CDNA4).
+//
+// One "line" (a row or a column of an N x N field) is transformed by T = N/8
+// threads that each hold 8 complex values in registers. The transform is a
+// Stockham autosort FFT: stage s (radix R, span Ns = product of the earlier
+// radices) takes x[b + r*N/R] for butterfly b, multiplies input r by
+// w^(r*(b mod Ns)) with w = exp(SIGN*2*pi*i/(Ns*R)), applies an R-point DFT and
+// writes y[(b/Ns)*Ns*R + (b mod Ns) + q*Ns]. Stages exchange data through LDS;
+// the first and the last stage are radix 8 so that thread t holds
+// x[t + r*T] on entry and X[t + q*T] on exit. A forward and an inverse transform
+// can therefore be chained in registers, with a per-element operation between
+// them, without touching LDS.
+//
+// Twiddles come from a per-workgroup LDS copy of root[m] = exp(-2*pi*i*m/N),
+// computed in double on the host and rounded to float.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace spr {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * (S*i), S = +1 or -1
+template <int S> __device__ __forceinline__ float2 mul_si(float2 a) {
+    return S > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <int S> __device__ __forceinline__ void dft2(float2& a0, float2& a1) {
+    const float2 t = a0;
+    a0 = cadd(t, a1);
+    a1 = csub(t, a1);
+}
+
+// X[k] = sum_n a[n] exp(S*2*pi*i*n*k/4)
+template <int S> __device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+    const float2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = mul_si<S>(csub(a1, a3));
+    a0 = cadd(t0, t2);
+    a2 = csub(t0, t2);
+    a1 = cadd(t1, t3);
+    a3 = csub(t1, t3);
+}
+
+// 8-point DFT as two 4-point DFTs (even/odd inputs) combined with w8^k.
+template <int S> __device__ __forceinline__ void dft8(float2* v) {
+    float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<S>(e0, e1, e2, e3);
+    dft4<S>(o0, o1, o2, o3);
+    const float h = 0.70710678118654752440f;
+    // w8 = (1 + S i)/sqrt2, w8^2 = S i, w8^3 = (-1 + S i)/sqrt2
+    const float2 p1 = make_float2((o1.x - S * o1.y) * h, (o1.y + S * o1.x) * h);
+    const float2 p2 = mul_si<S>(o2);
+    const float2 p3 = make_float2((-o3.x - S * o3.y) * h, (-o3.y + S * o3.x) * h);
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, p1);
+    v[5] = csub(e1, p1);
+    v[2] = cadd(e2, p2);
+    v[6] = csub(e2, p2);
+    v[3] = cadd(e3, p3);
+    v[7] = csub(e3, p3);
+}
+
+template <int R, int S> __device__ __forceinline__ void dftR(float2* v) {
+    if constexpr (R == 8) dft8<S>(v);
+    else if constexpr (R == 4) dft4<S>(v[0], v[1], v[2], v[3]);
+    else dft2<S>(v[0], v[1]);
+}
+
+// Radix plan: N = 8 * middle * 8 with middle in {1, 2, 4, 8, 16(=2x8), ...}.
+// Stage radices (first and last are 8).
+template <int N> struct RadixPlan;
+template <> struct RadixPlan<64>   { static constexpr int ns = 2; static constexpr int r[4] = {8, 8, 0, 0}; };
+template <> struct RadixPlan<128>  { static constexpr int ns = 3; static constexpr int r[4] = {8, 2, 8, 0}; };
+template <> struct RadixPlan<256>  { static constexpr int ns = 3; static constexpr int r[4] = {8, 4, 8, 0}; };
+template <> struct RadixPlan<512>  { static constexpr int ns = 3; static constexpr int r[4] = {8, 8, 8, 0}; };
+template <> struct RadixPlan<1024> { static constexpr int ns = 4; static constexpr int r[4] = {8, 2, 8, 8}; };
+template <> struct RadixPlan<2048> { static constexpr int ns = 4; static constexpr int r[4] = {8, 4, 8, 8}; };
+
+template <int N> constexpr int span_before(int s) {
+    int ns = 1;
+    for (int i = 0; i < s; ++i) ns *= RadixPlan<N>::r[i];
+    return ns;
+}
+
+// Twiddle and butterfly of stage STAGE on registers v (8 values = 8/R butterflies).
+template <int N, int STAGE, int SIGN>
+__device__ __forceinline__ void stage_compute(float2 (&v)[8], const float2* __restrict__ tw, int t) {
+    constexpr int R = RadixPlan<N>::r[STAGE];
+    constexpr int NS = span_before<N>(STAGE);
+    constexpr int T = N / 8;
+    constexpr int NB = 8 / R;
+    constexpr int STEP = N / (NS * R);
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        if constexpr (NS > 1) {
+            const int b = t + T * u;
+            const int k = b & (NS - 1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                float2 w = tw[r * k * STEP];
+                if (SIGN > 0) w.y = -w.y;
+                v[u * R + r] = cmul(v[u * R + r], w);
+            }
+        }
+        dftR<R, SIGN>(&v[u * R]);
+    }
+}
+
+// Stage inputs of stage STAGE from LDS: v[u*R + r] = x[b + r*N/R], b = t + T*u.
+template <int N, int STAGE, class Acc>
+__device__ __forceinline__ void stage_load(float2 (&v)[8], const Acc& acc, int t) {
+    constexpr int R = RadixPlan<N>::r[STAGE];
+    constexpr int T = N / 8;
+    constexpr int NB = 8 / R;
+    constexpr int M = N / R;
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[u * R + r] = acc.ld(t + T * u + r * M);
+}
+
+// Stage outputs of stage STAGE to LDS: y[(b/Ns)*Ns*R + b%Ns + q*Ns] = v[u*R + q].
+template <int N, int STAGE, class Acc>
+__device__ __forceinline__ void stage_store(const float2 (&v)[8], const Acc& acc, int t) {
+    constexpr int R = RadixPlan<N>::r[STAGE];
+    constexpr int NS = span_before<N>(STAGE);
+    constexpr int T = N / 8;
+    constexpr int NB = 8 / R;
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        const int b = t + T * u;
+        const int base = (b / NS) * NS * R + (b & (NS - 1));
+#pragma unroll
+        for (int q = 0; q < R; ++q) acc.st(base + q * NS, v[u * R + q]);
+    }
+}
+
+// Full line transform. On entry v[r] = x[t + r*N/8]; on exit v[q] = X[t + q*N/8]
+// with X[k] = sum_j x[j] exp(SIGN*2*pi*i*j*k/N) (unnormalised). Every thread of
+// the workgroup must call it (it contains workgroup barriers).
+template <int N, int SIGN, class Acc>
+__device__ __forceinline__ void fft_line(float2 (&v)[8], const Acc& acc, const float2* __restrict__ tw, int t) {
+    constexpr int NSTAGE = RadixPlan<N>::ns;
+    stage_compute<N, 0, SIGN>(v, tw, t);
+    stage_store<N, 0>(v, acc, t);
+    __syncthreads();
+    stage_load<N, 1>(v, acc, t);
+    stage_compute<N, 1, SIGN>(v, tw, t);
+    if constexpr (NSTAGE > 2) {
+        __syncthreads();
+        stage_store<N, 1>(v, acc, t);
+        __syncthreads();
+        stage_load<N, 2>(v, acc, t);
+        stage_compute<N, 2, SIGN>(v, tw, t);
+    }
+    if constexpr (NSTAGE > 3) {
+        __syncthreads();
+        stage_store<N, 2>(v, acc, t);
+        __syncthreads();
+        stage_load<N, 3>(v, acc, t);
+        stage_compute<N, 3, SIGN>(v, tw, t);
+    }
+    // Leave LDS free for the next user of this buffer.
+    __syncthreads();
+}
+
+}  // namespace spr

@@ -1,0 +1,224 @@
+// gfx950 kernels of the per-individual phase-retrieval iteration, batched over a
+// population of P individuals that share one detector amplitude.
+//
+// HBM layout (one context = one pattern):
+//   rho  [P][N][N] float2   real-space densities (the state between operations)
+//   work [P][N][N] float2   half-transformed fields (row-transformed / column-processed)
+//   supp [P][N][N] uint8    supports (0/1)
+//   amp  [N][N]    float    corner-origin amplitude with the mask encoding of the reference
+//   tw   [N]       float2   exp(-2*pi*i*m/N)
+//
+// One iteration = two launches (the minimal number of HBM round trips for a field
+// that does not fit in one CU's LDS):
+//   column pass  work <- colIFFT( P_M( colFFT(work) ) )                     16 B/px
+//   row pass     rho' = rowIFFT(work)/N^2 ; rho <- update(rho', rho, S) ;
+//                work <- rowFFT(rho)                                      33 B/px
+// so HIO/RAAR move 49 B/px per individual-iteration; ER, which needs no rho_n,
+// moves 33 B/px and writes rho only at the end.
+#pragma once
+
+#include <cstdint>
+
+#include "spr_fft.hpp"
+
+namespace spr {
+
+// ---------------------------------------------------------------------------
+// Launch geometry
+// ---------------------------------------------------------------------------
+template <int N> struct Geo {
+    static constexpr int T = N / 8;                       // threads per line
+    static constexpr int ROW_THREADS = 256;
+    static constexpr int RW = ROW_THREADS / T;            // rows per row-pass workgroup
+    static constexpr int RS = N + N / 8;                  // padded LDS row length (float2)
+    static constexpr int C = (8192 / N) < N ? (8192 / N) : N;  // columns per column-pass workgroup
+    static constexpr int COL_THREADS = C * T;
+    static constexpr int CS = C + 1;                      // padded LDS row length of a column tile
+    static_assert(RW >= 1 && ROW_THREADS % T == 0, "row geometry");
+    static_assert(COL_THREADS <= 1024, "column geometry");
+};
+
+struct RowAcc {
+    float2* base;
+    __device__ __forceinline__ float2 ld(int i) const { return base[i + (i >> 3)]; }
+    __device__ __forceinline__ void st(int i, float2 v) const { base[i + (i >> 3)] = v; }
+};
+
+template <int CS> struct ColAcc {
+    float2* base;
+    __device__ __forceinline__ float2 ld(int i) const { return base[i * CS]; }
+    __device__ __forceinline__ void st(int i, float2 v) const { base[i * CS] = v; }
+};
+
+// ---------------------------------------------------------------------------
+// Per-pixel operators (src/gpu/gpu_context.cu:79-111, 519-593)
+// ---------------------------------------------------------------------------
+
+// Reality window: the phase fails when atan2f(y,x) >= hi_ge or <= lo_le. The
+// float bounds are the nearest floats strictly beyond the reference's double
+// thresholds, so the float comparison is the reference's double comparison.
+// pi_only marks reality == 0, where only atan2f == +-pi_f can fail; the test
+// then skips atan2f for every value that is not (nearly) a negative real.
+struct PhaseCut {
+    float hi_ge, lo_le;
+    int pi_only;
+};
+
+__device__ __forceinline__ bool phase_fails(float x, float y, const PhaseCut& pc) {
+    if (pc.pi_only) {
+        if (!signbit(x) || fabsf(y) > 1e-6f * fabsf(x)) return false;
+    }
+    const float ph = atan2f(y, x);
+    return ph >= pc.hi_ge || ph <= pc.lo_le;
+}
+
+// Fourier modulus projection P_M (sub_intensities).
+__device__ __forceinline__ float2 project_modulus(float2 F, float a) {
+    const float m = sqrtf(F.y * F.y + F.x * F.x);
+    if (a >= 0.f) {
+        if (m > 0.f) return make_float2(a * F.x / m, a * F.y / m);
+        const float ph = atan2f(F.y, F.x);
+        return make_float2(a * cosf(ph), a * sinf(ph));
+    }
+    if (a < -1.5f && m > -a - 2.f) {
+        const float nm = -a - 2.f;
+        return make_float2(nm * F.x / m, nm * F.y / m);
+    }
+    return F;
+}
+
+enum Alg { ALG_ER = 0, ALG_HIO = 1, ALG_RAAR = 2 };
+
+// ---------------------------------------------------------------------------
+// Row pass
+// ---------------------------------------------------------------------------
+enum RowMode {
+    ROW_FWD = 0,      // work <- rowFFT+(rho)
+    ROW_FINAL = 1,    // rho <- rowFFT-(work)/N^2
+    ROW_HIO = 2,      // rho' = rowFFT-(work)/N^2; rho <- HIO; work <- rowFFT+(rho)
+    ROW_RAAR = 3,
+    ROW_ER = 4,       // as HIO without rho_n; rho is not written
+    ROW_FWD_INPLACE = 5,   // rho <- rowFFT+(rho)   (2-D FT, first half)
+    ROW_INV_INPLACE = 6,   // rho <- rowFFT-(rho)/N^2 (2-D IFT, second half)
+};
+
+struct RowArgs {
+    float2* rho;
+    float2* work;
+    const uint8_t* supp;
+    const float2* tw;
+    float beta;
+    float inv_nn;  // 1/N^2 (exact for power-of-two N)
+    PhaseCut pc;
+};
+
+template <int N, int MODE>
+__global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
+    using G = Geo<N>;
+    constexpr int T = G::T;
+    __shared__ float2 tw[N];
+    __shared__ float2 lds[G::RW * G::RS];
+    const int tid = threadIdx.x;
+    const int t = tid % T;
+    const int rl = tid / T;
+    for (int i = tid; i < N; i += G::ROW_THREADS) tw[i] = a.tw[i];
+    const size_t base = ((size_t)blockIdx.x * G::RW + rl) * N;
+    const RowAcc acc{lds + rl * G::RS};
+    float2 v[8];
+    __syncthreads();
+
+    if constexpr (MODE == ROW_FWD || MODE == ROW_FWD_INPLACE) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = a.rho[base + t + r * T];
+        fft_line<N, +1>(v, acc, tw, t);
+        float2* dst = (MODE == ROW_FWD) ? a.work : a.rho;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) dst[base + t + q * T] = v[q];
+    } else if constexpr (MODE == ROW_FINAL || MODE == ROW_INV_INPLACE) {
+        const float2* src = (MODE == ROW_FINAL) ? a.work : a.rho;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = src[base + t + r * T];
+        fft_line<N, -1>(v, acc, tw, t);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) a.rho[base + t + q * T] = make_float2(v[q].x * a.inv_nn, v[q].y * a.inv_nn);
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = a.work[base + t + r * T];
+        fft_line<N, -1>(v, acc, tw, t);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const size_t i = base + t + q * T;
+            const float2 d = make_float2(v[q].x * a.inv_nn, v[q].y * a.inv_nn);  // rho'
+            const uint8_t s = a.supp[i];
+            float2 o;
+            if constexpr (MODE == ROW_ER) {
+                const float ts = (s != 0 && !phase_fails(d.x, d.y, a.pc)) ? 1.f : 0.f;
+                o = make_float2(d.x * ts, d.y * ts);
+            } else if constexpr (MODE == ROW_HIO) {
+                const float2 b = a.rho[i];
+                const float ts = (s != 0 && !phase_fails(d.x, d.y, a.pc)) ? 1.f : 0.f;
+                o = make_float2((b.x - a.beta * d.x) * (1.f - ts) + d.x * ts,
+                                (b.y - a.beta * d.y) * (1.f - ts) + d.y * ts);
+            } else {  // RAAR
+                const float2 b = a.rho[i];
+                const float qx = 2.f * d.x - b.x, qy = 2.f * d.y - b.y;
+                o = d;
+                if (s == 0 || phase_fails(qx, qy, a.pc)) {
+                    const float c = 1.f - 2.f * a.beta;
+                    o = make_float2(a.beta * b.x + c * d.x, a.beta * b.y + c * d.y);
+                }
+            }
+            if constexpr (MODE != ROW_ER) a.rho[i] = o;
+            v[q] = o;
+        }
+        fft_line<N, +1>(v, acc, tw, t);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) a.work[base + t + q * T] = v[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Column pass
+// ---------------------------------------------------------------------------
+enum ColMode {
+    COL_PROJ = 0,   // colIFFT(P_M(colFFT(.)))
+    COL_FWD = 1,    // colFFT+(.)
+    COL_INV = 2,    // colFFT-(.)
+};
+
+struct ColArgs {
+    float2* f;          // field batch, transformed in place
+    const float* amp;
+    const float2* tw;
+};
+
+template <int N, int MODE>
+__global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col(ColArgs a) {
+    using G = Geo<N>;
+    constexpr int T = G::T, C = G::C;
+    __shared__ float2 tw[N];
+    __shared__ float2 lds[N * G::CS];
+    const int tid = threadIdx.x;
+    const int c = tid % C;
+    const int t = tid / C;
+    for (int i = tid; i < N; i += G::COL_THREADS) tw[i] = a.tw[i];
+    constexpr int TILES = N / C;
+    const int p = blockIdx.x / TILES;
+    const int x0 = (blockIdx.x % TILES) * C;
+    const size_t base = (size_t)p * N * N + x0 + c;
+    const ColAcc<G::CS> acc{lds + c};
+    float2 v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = a.f[base + (size_t)(t + r * T) * N];
+    __syncthreads();
+    if constexpr (MODE == COL_PROJ || MODE == COL_FWD) fft_line<N, +1>(v, acc, tw, t);
+    if constexpr (MODE == COL_PROJ) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = project_modulus(v[q], a.amp[(size_t)(t + q * T) * N + x0 + c]);
+    }
+    if constexpr (MODE == COL_PROJ || MODE == COL_INV) fft_line<N, -1>(v, acc, tw, t);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) a.f[base + (size_t)(t + q * T) * N] = v[q];
+}
+
+}  // namespace spr
