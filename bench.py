@@ -69,6 +69,25 @@ def cpu_baseline(amp, rho, supp, alg, beta, iters, log):
             "sample": f"FP32 oracle, {P} individuals x {iters} {alg} iterations at {amp.shape[0]}^2 ({cpu})"}
 
 
+def copy_calibration(mib, log):
+    """Three device-to-device copies of `mib` MiB through the HIP runtime's plain
+    copy kernel, so that a counter run (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
+    records what the counters read for a known number of bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    for _ in range(3):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3):   # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: 3 device-to-device copies of {mib} MiB")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +100,8 @@ def main():
     ap.add_argument("--beta", type=float, default=0.87)
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", type=int, default=0, metavar="MIB",
+                    help="also run device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
 
     def log(msg):
@@ -104,6 +125,8 @@ def main():
 
     be = hipcore.Backend(amp, capacity=P, device=0)
     be.set_population(rho, supps)
+    if args.calibrate:
+        copy_calibration(args.calibrate, log)
     log("warmup")
     for _ in range(args.warmup):
         be.iterate_async(args.alg, args.block, args.beta, 0.0)

@@ -53,6 +53,12 @@ enum spr_alg { SPR_ER = 0, SPR_HIO = 1, SPR_RAAR = 2 };
 /* Supported grid sizes N (spr_supported_size(N) != 0). */
 int spr_supported_size(int n);
 
+/* Device enumeration, replacing GPU_utils::get_count / get_info
+ * (src/gpu/gpu_utils.cu:13-111): number of visible devices (0 without a GPU)
+ * and the name of one. */
+int spr_device_count(int* n);
+int spr_device_name(int device, char* buf, int len);
+
 /* Creates a backend for one n x n pattern on `device` with room for `capacity`
  * individuals; uploads the corner-origin amplitude `amp` (n*n floats).
  * Replaces GPU_context::GPU_context (src/gpu/gpu_context.h:17-84), which the
@@ -97,6 +103,106 @@ int spr_iterate(spr_ctx* ctx, int alg, int niter, float beta, float reality);
  * handle's stream (for timing loops); spr_sync waits for it. */
 int spr_iterate_async(spr_ctx* ctx, int alg, int niter, float beta, float reality);
 int spr_sync(spr_ctx* ctx);
+
+/* Context::SW (src/context.h:49, src/gpu/gpu_context.cu:906-953) on every
+ * resident individual: S <- [|b| >= max|b| * threshold], with b the Gaussian
+ * (sigma) blur of |IFT(P_M(FT rho))|. The densities are not changed. `reality`
+ * is accepted for signature parity and unused, as in the reference. */
+int spr_sw(spr_ctx* ctx, float sigma, float threshold, float reality);
+
+/* Context::SWA (src/context.h:50, src/gpu/gpu_context.cu:957-1027): as
+ * spr_sw, with the threshold set to the `spixels`-th largest |b| of each
+ * individual (exact selection on the device; the reference sorts on the host).
+ * spixels <= 0 yields empty supports (out of bounds in the reference). */
+int spr_swa(spr_ctx* ctx, float sigma, int spixels, float reality);
+
+/* Context::get_error + get_symmetry(sym, 1, 0, 1) for every resident
+ * individual (src/gpu/gpu_context.cu:1030-1134, 1378-1463), as called by
+ * Solver::Sequence / Evaluate (src/solver.cpp:474-475, 549-550). Outputs are
+ * arrays of P floats; any may be NULL (symmetry == NULL skips its cost).
+ * Sums are accumulated in FP64 in a fixed order. */
+int spr_evaluate(spr_ctx* ctx, float reality, float* error_fourier, float* error_real, float* complexsum,
+                 float* modulesum, float* symmetry);
+
+/* Context::get_symmetry(sym, 1, symmin, symmax) for every resident individual. */
+int spr_symmetry(spr_ctx* ctx, float symmin, float symmax, float* symmetry);
+
+/* One operation of an algorithm sequence (the closures Solver::get_algorithm
+ * builds, src/solver.cpp:84-159). */
+enum spr_op_kind { SPR_OP_ER = 0, SPR_OP_HIO = 1, SPR_OP_RAAR = 2, SPR_OP_SW = 3, SPR_OP_SWA = 4, SPR_OP_GRAD = 5 };
+typedef struct spr_op {
+    int kind;          /* spr_op_kind */
+    int iterations;    /* ER/HIO/RAAR: iterations; GRAD: Adam steps */
+    float beta;        /* HIO/RAAR: beta; GRAD: learning rate */
+    float sigma;       /* SW/SWA */
+    float threshold;   /* SW: threshold; GRAD: loss eps */
+    int pixels;        /* SWA: support pixels; GRAD: loss kind (spr_loss_grad) */
+} spr_op;
+
+/* The device half of Solver::Sequence (src/solver.cpp:454-487): nrep
+ * repetitions of the op list on every resident individual, queued on the
+ * handle's stream and synchronised once at the end. */
+int spr_run(spr_ctx* ctx, const spr_op* ops, int nops, int nrep, float reality);
+
+/* Solver::Reshift on the device (src/solver.cpp:511-538): for every resident
+ * individual, set_ref_density(ref) + get_shift(minr, maxr) + reshift
+ * (src/gpu/gpu_context.cu:632-667, 1226-1371). ref_rho: the reference
+ * density (n*n complex, host). phase_direct[p] / phase_flip[p]: the shift
+ * phases the reference forms from float sums of the host densities
+ * (density_ref_phase -/+ density_phase, :1309-1321). shifts (optional, 3*P
+ * int32): the chosen (shiftx, shifty, flip) per individual. */
+int spr_align(spr_ctx* ctx, const float* ref_rho, float minr, float maxr, const float* phase_direct,
+              const float* phase_flip, int32_t* shifts);
+
+/* Solver::GetAverage on the device (src/solver.cpp:618-686): the mean density
+ * of the resident individuals listed in idx (navg entries, summed in that
+ * order) and the union of their supports. rho_out: n*n complex, supp_out: n*n int32. */
+int spr_average(spr_ctx* ctx, const int32_t* idx, int navg, float* rho_out, int32_t* supp_out);
+
+/* Solver::GetAverageModule (src/solver.cpp:564-614): mean of |FT rho| (ft != 0)
+ * or of |rho| over the listed individuals. out: n*n floats. */
+int spr_average_module(spr_ctx* ctx, const int32_t* idx, int navg, int ft, float* out);
+
+/* One child of Solver::Crossover (src/solver.cpp:742-814): parents are
+ * resident individuals; p0 is the individual the child replaces. The random
+ * draws are the caller's (host RNG, as in the reference). */
+typedef struct spr_cross_desc {
+    int p0, p1, p2, p3;
+    int crossft, crosssym;
+    int crosssizex, crosssizey, squaresizex, squaresizey, offsetx, offsety;
+    float rotangle;
+    int crossmap_offset;   /* first entry of this child's crossmap in the concatenated array */
+    /* alignment phases to p1 (reshift_parent, src/gpu/gpu_context.cu:1536-1654) of
+     * the roles p0, p2, p3, avg: {direct, flip} each */
+    float phase[8];
+} spr_cross_desc;
+
+/* prepare_parents + get_new_individual (src/gpu/gpu_context.cu:1661-1694,
+ * 2044-2117) for nchild children at once. avg_rho: the population average
+ * density (n*n complex); amp_avg: GetAverageModule(ft, 0.66) (n*n floats).
+ * Outputs: child densities (nchild*n*n complex) and supports (nchild*n*n int32;
+ * with update_support == 0 they are p0's supports). */
+int spr_crossover(spr_ctx* ctx, int nchild, const spr_cross_desc* desc, const int32_t* crossmaps, int crossmap_len,
+                  const float* avg_rho, const float* amp_avg, float crossdiff, float use_average, float minr,
+                  float maxr, float reality, int update_support, float* child_rho, int32_t* child_supp);
+
+/* Masked Fourier-magnitude loss and its gradient (no counterpart in the
+ * reference; the gradient-refinement operator of the memetic loop). For
+ * F = FT(rho) and the measured pixels (A >= 0, counts I = A^2), u = |F|^2:
+ *   kind 0 (Poisson NLL)  L = sum u - I log(u + eps)
+ *   kind 1 (amplitude)    L = sum (|F| - A)^2
+ * grad = dL/dRe(rho) + i dL/dIm(rho) = 2 FT^H[f'(u) F] (PyTorch's convention for
+ * complex tensors). rho and grad are device arrays of P fields (P <= capacity);
+ * loss (optional, host, P doubles, FP64 sums). Runs on `stream` (a hipStream_t;
+ * NULL: the handle's stream) and returns after the work is queued unless loss
+ * is requested. */
+int spr_loss_grad(spr_ctx* ctx, int P, const float* rho, float* grad, double* loss, int kind, float eps,
+                  void* stream);
+
+/* `steps` Adam steps (lr; beta 0.9/0.999, eps 1e-8, moments reset per call)
+ * of the loss above on every resident density. loss (optional, host, P
+ * doubles): the loss after the last step. */
+int spr_refine(spr_ctx* ctx, int steps, float lr, int kind, float eps, double* loss);
 
 /* Device addresses of the resident population: rho (P*n*n float2) and the
  * uint8 supports; for callers that fill or read HBM directly. */

@@ -44,8 +44,28 @@ def build_hip(verbose: bool = False) -> str:
     return out
 
 
+def build_core(verbose: bool = False) -> str:
+    """pybind11 module spring_core (host C++: Solver + bindings), linked against
+    lib/libspring_hip.so. Built with g++ (no device code)."""
+    import sysconfig
+
+    import pybind11
+    suffix = sysconfig.get_config_var("EXT_SUFFIX")
+    out = os.path.join(HERE, "spring_core" + suffix)
+    tmp = out + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
+           "-I", INCLUDE, "-I", CSRC, "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"],
+           os.path.join(CSRC, "solver.cpp"), os.path.join(CSRC, "spring_core.cpp"),
+           "-L", LIBDIR, "-lspring_hip", "-Wl,-rpath,$ORIGIN/lib", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
 def build_all(verbose: bool = False):
-    return [build_hip(verbose)]
+    return [build_hip(verbose), build_core(verbose)]
 
 
 if __name__ == "__main__":

@@ -1,0 +1,102 @@
+// Solver: population-level operations of memetic phase retrieval, with the
+// signatures of the reference's Solver (src/solver.h:17-91). Where the
+// reference loops over individuals with one CUDA context per OpenMP thread
+// (src/solver.cpp:360-487), this Solver hands the whole population to one
+// device backend (include/spring_hip.h) per call.
+#pragma once
+
+#include <array>
+#include <functional>
+#include <memory>
+#include <random>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "reconstruction.h"
+#include "spring_hip.h"
+
+namespace spring {
+
+using RecPtr = std::shared_ptr<Reconstruction>;
+using RecVec = std::vector<RecPtr>;
+using SeqItem = std::tuple<std::string, std::vector<FLOAT>>;
+
+// Random stream of one reference Context (src/context.h:13-18, 68-69):
+// std::mt19937 seeded with (i+1)*seed and a float uniform distribution on [0,1).
+struct ContextRng {
+    std::mt19937 generator;
+    std::uniform_real_distribution<FLOAT> distribution{0.f, 1.f};
+    explicit ContextRng(int seed) : generator((std::mt19937::result_type)seed) {}
+    FLOAT rand(FLOAT min, FLOAT max) { return distribution(generator) * (max - min) + min; }
+    FLOAT rand() { return distribution(generator); }
+};
+
+class Solver {
+public:
+    Solver(const FLOAT* pattern_in, std::array<std::size_t, 2> dims_in, std::array<int, 2> center_in, FLOAT reality_,
+           FLOAT realpen_, int nthreads_, bool use_bounds_, std::vector<int> GPU_ids, int seed);
+    ~Solver();
+    Solver(const Solver&) = delete;
+    Solver& operator=(const Solver&) = delete;
+
+    void prepare(bool use_bounds);
+    void set_boundaries();
+    void set_threads(int nthreads_ = 0);
+
+    RecVec HIO(RecVec rec, int iterations, FLOAT beta);
+    RecVec RAAR(RecVec rec, int iterations, FLOAT beta);
+    RecVec ER(RecVec rec, int iterations);
+    RecVec SW(RecVec rec, FLOAT sigma, FLOAT threshold);
+    RecVec SWA(RecVec rec, FLOAT sigma, int pixels);
+
+    void Evaluate(RecVec rec);
+    RecVec Sequence(RecVec rec, int nrep);
+    RecVec Reshift(RecVec rec, RecPtr ref);
+    RecVec Crossover(RecVec rec, RecPtr best, RecPtr avg, FLOAT crossbalance, FLOAT crossdiff, FLOAT crossexp,
+                     FLOAT crossft, bool crosssym, FLOAT use_average, bool update_support);
+
+    std::vector<FLOAT> get_pattern();
+    void set_pattern(const FLOAT* pattern_in);
+    void set_sequence(const std::vector<SeqItem>& sequence);
+    void set_reality(FLOAT r) { reality = r; }
+    void set_shift_param(FLOAT minr, FLOAT maxr) {
+        shiftmin = minr;
+        shiftmax = maxr;
+    }
+
+    RecPtr GetAverage(RecVec rec, FLOAT fraction);
+    std::vector<FLOAT> GetAverageModule(RecVec rec, bool ft, FLOAT fraction);
+
+    RecVec Initialize_pop(int nrec, int support_size, std::array<FLOAT, 2> itemsize, std::array<int, 2> nitems,
+                          std::array<FLOAT, 2> exponent, std::array<FLOAT, 2> scaling, FLOAT phase, FLOAT suppcut);
+    RecVec Initialize_pop(int nrec, RecPtr refrec, std::array<FLOAT, 2> itemsize, std::array<int, 2> nitems,
+                          std::array<FLOAT, 2> exponent, std::array<FLOAT, 2> scaling, FLOAT phase, FLOAT suppcut);
+    RecVec Initialize_pop(int nrec, std::vector<SUPPORTTYPE> refsupport, FLOAT initfrac, std::array<FLOAT, 2> scaling,
+                          FLOAT phase);
+
+    std::vector<FLOAT> pattern;          // corner-origin amplitude with the mask encoding
+    std::array<std::size_t, 2> dims;
+    std::array<int, 2> center;
+    std::size_t size;
+    FLOAT reality, realpen, shiftmin, shiftmax;
+    int nthreads;
+    std::vector<spr_op> ops;             // the op list of set_sequence
+
+private:
+    spr_ctx* backend(int capacity);
+    void upload(const RecVec& rec, bool support = true);
+    void download(const RecVec& rec, bool support = true);
+    void run_op(const RecVec& rec, const spr_op& op);
+    void evaluate_resident(const RecVec& rec);
+    void check(int rc) const;
+    std::vector<int> sorted_by_error(const RecVec& rec) const;
+    std::vector<COMPLEX> ft_batch(const std::vector<std::vector<COMPLEX>>& fields, bool inverse);
+
+    std::vector<ContextRng> rngs;       // one random stream per reference context
+    int device;
+    spr_ctx* ctx = nullptr;
+    int ctx_capacity = 0;
+};
+
+}  // namespace spring

@@ -2,6 +2,7 @@
 // declared in include/spring_hip.h. Kernels are in spr_kernels.hpp.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -9,7 +10,7 @@
 #include <vector>
 
 #include "spring_hip.h"
-#include "spr_kernels.hpp"
+#include "spr_grad.hpp"
 
 using namespace spr;
 
@@ -18,9 +19,24 @@ struct spr_ctx {
     hipStream_t stream = nullptr;
     float2* rho = nullptr;
     float2* work = nullptr;
+    float2* work2 = nullptr;      // second half-transformed field per individual (correlations, |b|)
     uint8_t* supp = nullptr;
     float* amp = nullptr;
     float2* tw = nullptr;
+    // reduction scratch
+    double* partial = nullptr;    // per workgroup partial sums
+    double* sums = nullptr;       // [cap][SUMS_PER_ITEM]
+    unsigned* maxbits = nullptr;  // [cap][2]
+    unsigned long long* keys = nullptr;  // [cap][2]
+    unsigned* radix = nullptr;    // [3][cap]: prefix, mask, remaining
+    unsigned* hist = nullptr;     // [cap][2048]
+    size_t partial_cap = 0;       // doubles
+    // Alignment / averaging / crossover workspace, grown on demand.
+    struct Buf {
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+    Buf spec, xf, xs, supp2, fieldbuf, ampavg, xidx, shiftp, xchild, xmaps, xphase, adam_m, adam_v;
     std::string err;
     // Opt-in per-launch timing: events bracket every launch on the stream.
     bool timing = false;
@@ -177,6 +193,222 @@ RowArgs row_args(spr_ctx* c) {
         if (rc_ != SPR_OK) return rc_; \
     } while (0)
 
+// ---------------------------------------------------------------------------
+// Auxiliary passes (shrink-wrap, fitness, symmetry, correlations)
+// ---------------------------------------------------------------------------
+constexpr int SUMS_PER_ITEM = 8;   // msum, err, nrm, rerr, rnrm
+constexpr int RADIX_BINS = 2048;
+
+template <int N> int launch_row_aux(spr_ctx* c, int mode, const AuxRowArgs& a, int items) {
+    const dim3 grid((unsigned)((size_t)items * N / Geo<N>::RW)), block(Geo<N>::ROW_THREADS);
+    switch (mode) {
+        case AROW_ABS_FWD_WORK: hipLaunchKernelGGL((k_row_aux<N, AROW_ABS_FWD_WORK>), grid, block, 0, c->stream, a); break;
+        case AROW_MAG_MAX: hipLaunchKernelGGL((k_row_aux<N, AROW_MAG_MAX>), grid, block, 0, c->stream, a); break;
+        case AROW_ERR_FWD: hipLaunchKernelGGL((k_row_aux<N, AROW_ERR_FWD>), grid, block, 0, c->stream, a); break;
+        case AROW_ERR_REAL: hipLaunchKernelGGL((k_row_aux<N, AROW_ERR_REAL>), grid, block, 0, c->stream, a); break;
+        case AROW_ABS_FWD_RHO: hipLaunchKernelGGL((k_row_aux<N, AROW_ABS_FWD_RHO>), grid, block, 0, c->stream, a); break;
+        case AROW_CORR: hipLaunchKernelGGL((k_row_aux<N, AROW_CORR>), grid, block, 0, c->stream, a); break;
+        default: return fail(c, SPR_ERR_ARG, "bad aux row mode");
+    }
+    SPR_HIP(c, hipGetLastError());
+    return SPR_OK;
+}
+
+template <int N> int launch_col_aux(spr_ctx* c, int mode, const AuxColArgs& a, int items) {
+    const dim3 grid((unsigned)((size_t)items * (N / Geo<N>::C))), block(Geo<N>::COL_THREADS);
+    switch (mode) {
+        case ACOL_BLUR: hipLaunchKernelGGL((k_col_aux<N, ACOL_BLUR>), grid, block, 0, c->stream, a); break;
+        case ACOL_ERR: hipLaunchKernelGGL((k_col_aux<N, ACOL_ERR>), grid, block, 0, c->stream, a); break;
+        case ACOL_CORR: hipLaunchKernelGGL((k_col_aux<N, ACOL_CORR>), grid, block, 0, c->stream, a); break;
+        case ACOL_FWD_UNIT: hipLaunchKernelGGL((k_col_aux<N, ACOL_FWD_UNIT>), grid, block, 0, c->stream, a); break;
+        default: return fail(c, SPR_ERR_ARG, "bad aux column mode");
+    }
+    SPR_HIP(c, hipGetLastError());
+    return SPR_OK;
+}
+
+int row_aux(spr_ctx* c, int mode, const AuxRowArgs& a, int items) {
+    SPR_DISPATCH_N(c, launch_row_aux, c, mode, a, items);
+}
+int col_aux(spr_ctx* c, int mode, const AuxColArgs& a, int items) {
+    SPR_DISPATCH_N(c, launch_col_aux, c, mode, a, items);
+}
+
+int rows_per_wg(int n) { return 256 / (n / 8); }
+int cols_per_wg(int n) { return (8192 / n) < n ? (8192 / n) : n; }
+
+AuxRowArgs aux_row_args(spr_ctx* c) {
+    AuxRowArgs a{};
+    a.src = c->rho;
+    a.work = c->work;
+    a.work2 = c->work2;
+    a.supp = c->supp;
+    a.tw = c->tw;
+    a.mag = reinterpret_cast<float*>(c->work2);
+    a.maxbits = c->maxbits;
+    a.key1 = c->keys;
+    a.key2 = c->keys + c->cap;
+    a.partial = c->partial;
+    a.inv_nn = 1.0f / (float)((double)c->n * c->n);
+    a.pc = cut_er_hio(0.f);
+    a.src_index = nullptr;
+    return a;
+}
+AuxColArgs aux_col_args(spr_ctx* c) {
+    AuxColArgs a{};
+    a.f = c->work;
+    a.f2 = c->work2;
+    a.amp = c->amp;
+    a.tw = c->tw;
+    a.partial = c->partial;
+    a.maxbits = c->maxbits;
+    return a;
+}
+
+// The reference forms the window constants in float/double mixes
+// (src/gpu/gpu_context.cu:1171-1180); reproduced here on the host.
+CorrWin make_corr_win(int n, float minr, float maxr) {
+    CorrWin w;
+    const float sigmatot = (float)((double)((float)n * minr) / 2. + 1e-12);
+    const float sigma = (float)((double)sigmatot / 10.);
+    const float sigmatotmax = (float)((double)((float)n * maxr) / 2. + 1e-12);
+    const float sigmamax = (float)((double)sigmatotmax / 10.);
+    w.sigtot_minus_sig = sigmatot - sigma;
+    w.sig2 = sigma * sigma;
+    w.sigtotmax_minus_sigmax = sigmatotmax - sigmamax;
+    w.sigmax2 = sigmamax * sigmamax;
+    return w;
+}
+
+int fill_u32(spr_ctx* c, unsigned* p, unsigned v, int n) {
+    if (n <= 0) return SPR_OK;
+    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, c->stream, p, v, n);
+    SPR_HIP(c, hipGetLastError());
+    return SPR_OK;
+}
+int fill_u64(spr_ctx* c, unsigned long long* p, unsigned long long v, int n) {
+    if (n <= 0) return SPR_OK;
+    hipLaunchKernelGGL(k_fill_u64, dim3((n + 255) / 256), dim3(256), 0, c->stream, p, v, n);
+    SPR_HIP(c, hipGetLastError());
+    return SPR_OK;
+}
+int reduce(spr_ctx* c, int per_item, int K, int out_off, int items) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(items), dim3(256), 0, c->stream, (const double*)c->partial, per_item,
+                       K, c->sums, SUMS_PER_ITEM, out_off, items);
+    SPR_HIP(c, hipGetLastError());
+    return SPR_OK;
+}
+
+#define SPR_TRY2(expr)                 \
+    do {                               \
+        int rc_ = (expr);              \
+        if (rc_ != SPR_OK) return rc_; \
+    } while (0)
+
+// |b| of shrink-wrap into mag = (float*)work2 and max|b| into maxbits[0..P)
+// (src/gpu/gpu_context.cu:911-944).
+int queue_blur_magnitude(spr_ctx* c, float sigma, bool want_max) {
+    RowArgs ra = row_args(c);
+    SPR_TRY2(row(c, ROW_FWD, ra));
+    SPR_TRY2(col(c, COL_PROJ, c->work));
+    AuxRowArgs a = aux_row_args(c);
+    SPR_TRY2(row_aux(c, AROW_ABS_FWD_WORK, a, c->P));
+    AuxColArgs ca = aux_col_args(c);
+    ca.sigma_ft = (float)((double)((float)c->n / sigma) / M_PI);
+    SPR_TRY2(col_aux(c, ACOL_BLUR, ca, c->P));
+    if (want_max) SPR_TRY2(fill_u32(c, c->maxbits, 0u, c->P));
+    SPR_TRY2(row_aux(c, AROW_MAG_MAX, a, c->P));
+    return SPR_OK;
+}
+
+int queue_sw(spr_ctx* c, float sigma, float threshold) {
+    if (c->P <= 0) return SPR_OK;
+    SPR_TRY2(queue_blur_magnitude(c, sigma, true));
+    const size_t total = (size_t)c->n * c->n * c->P;
+    hipLaunchKernelGGL(k_cut_max, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->stream,
+                       (const float*)c->work2, (const unsigned*)c->maxbits, threshold, c->supp, c->n * c->n, c->P);
+    SPR_HIP(c, hipGetLastError());
+    return SPR_OK;
+}
+
+int queue_swa(spr_ctx* c, float sigma, int spixels) {
+    if (c->P <= 0) return SPR_OK;
+    const int npix = c->n * c->n;
+    SPR_TRY2(queue_blur_magnitude(c, sigma, true));
+    unsigned* prefix = c->radix;
+    unsigned* mask = c->radix + c->cap;
+    unsigned* rem = c->radix + 2 * c->cap;
+    const unsigned want = spixels <= 0 ? 0u : (unsigned)(spixels > npix ? npix : spixels);
+    SPR_TRY2(fill_u32(c, prefix, 0u, c->P));
+    SPR_TRY2(fill_u32(c, mask, 0u, c->P));
+    SPR_TRY2(fill_u32(c, rem, want, c->P));
+    const int bpi = 16;   // histogram blocks per individual
+    const int los[3] = {21, 10, 0};
+    for (int pass = 0; pass < 3; ++pass) {
+        SPR_TRY2(fill_u32(c, c->hist, 0u, c->P * RADIX_BINS));
+        const int lo = los[pass];
+        if (pass < 2) {
+            hipLaunchKernelGGL((k_radix_hist<11>), dim3(c->P * bpi), dim3(256), 0, c->stream, (const float*)c->work2,
+                               npix, lo, (const unsigned*)prefix, (const unsigned*)mask, c->hist, bpi);
+            SPR_HIP(c, hipGetLastError());
+            hipLaunchKernelGGL((k_radix_pick<11>), dim3((c->P + 63) / 64), dim3(64), 0, c->stream,
+                               (const unsigned*)c->hist, lo, prefix, mask, rem, c->P);
+        } else {
+            hipLaunchKernelGGL((k_radix_hist<10>), dim3(c->P * bpi), dim3(256), 0, c->stream, (const float*)c->work2,
+                               npix, lo, (const unsigned*)prefix, (const unsigned*)mask, c->hist, bpi);
+            SPR_HIP(c, hipGetLastError());
+            hipLaunchKernelGGL((k_radix_pick<10>), dim3((c->P + 63) / 64), dim3(64), 0, c->stream,
+                               (const unsigned*)c->hist, lo, prefix, mask, rem, c->P);
+        }
+        SPR_HIP(c, hipGetLastError());
+    }
+    const size_t total = (size_t)npix * c->P;
+    hipLaunchKernelGGL(k_cut_bits, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->stream,
+                       (const float*)c->work2, (const unsigned*)prefix, c->supp, npix, c->P);
+    SPR_HIP(c, hipGetLastError());
+    return SPR_OK;
+}
+
+// get_error: sums[p] = {msum, err, nrm, rerr, rnrm}, maxbits[p] = max|F|.
+int queue_error(spr_ctx* c, float reality) {
+    const int n = c->n;
+    AuxRowArgs a = aux_row_args(c);
+    a.pc = cut_er_hio(reality);
+    SPR_TRY2(row_aux(c, AROW_ERR_FWD, a, c->P));
+    SPR_TRY2(reduce(c, n / rows_per_wg(n), 1, 0, c->P));
+    AuxColArgs ca = aux_col_args(c);
+    SPR_TRY2(fill_u32(c, c->maxbits, 0u, c->P));
+    SPR_TRY2(col_aux(c, ACOL_ERR, ca, c->P));
+    SPR_TRY2(reduce(c, n / cols_per_wg(n), 2, 1, c->P));
+    SPR_TRY2(row_aux(c, AROW_ERR_REAL, a, c->P));
+    SPR_TRY2(reduce(c, n / rows_per_wg(n), 2, 3, c->P));
+    return SPR_OK;
+}
+
+// Symmetry correlation maxima into keys (key1: conj(a) a, key2: a a).
+int queue_symmetry(spr_ctx* c, float symmin, float symmax) {
+    AuxRowArgs a = aux_row_args(c);
+    a.src = c->rho;
+    SPR_TRY2(row_aux(c, AROW_ABS_FWD_RHO, a, c->P));
+    AuxColArgs ca = aux_col_args(c);
+    ca.win = make_corr_win(c->n, symmin, symmax);
+    ca.unit_scale = 1;
+    ca.ref = nullptr;
+    ca.ref_index = nullptr;
+    SPR_TRY2(col_aux(c, ACOL_CORR, ca, c->P));
+    SPR_TRY2(fill_u64(c, c->keys, 0ull, 2 * c->cap));
+    SPR_TRY2(row_aux(c, AROW_CORR, a, c->P));
+    return SPR_OK;
+}
+
+float key_value(unsigned long long k) {
+    const unsigned b = (unsigned)(k >> 32);
+    float f;
+    std::memcpy(&f, &b, 4);
+    return k ? f : 0.f;
+}
+int key_index(unsigned long long k) { return k ? (int)(0xFFFFFFFFu - (unsigned)(k & 0xFFFFFFFFu)) : 0; }
+
 int queue_iterate(spr_ctx* c, int alg, int niter, float beta, float reality) {
     if (alg < SPR_ER || alg > SPR_RAAR) return fail(c, SPR_ERR_ARG, "unknown algorithm");
     if (niter < 0) return fail(c, SPR_ERR_ARG, "negative iteration count");
@@ -193,6 +425,147 @@ int queue_iterate(spr_ctx* c, int alg, int niter, float beta, float reality) {
     }
     SPR_TRY(col(c, COL_PROJ, c->work));
     SPR_TRY(row(c, ROW_FINAL, a));
+    return SPR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Workspace and batched helpers of alignment, averages and crossover
+// ---------------------------------------------------------------------------
+int ensure(spr_ctx* c, spr_ctx::Buf& b, size_t bytes) {
+    if (b.bytes >= bytes) return SPR_OK;
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    if (b.p) SPR_HIP(c, hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+    SPR_HIP(c, hipMalloc(&b.p, bytes));
+    b.bytes = bytes;
+    return SPR_OK;
+}
+
+template <int N> int launch_fft_rows(spr_ctx* c, int sign, FieldBatch b, int items) {
+    const dim3 grid((unsigned)((size_t)items * N / Geo<N>::RW)), block(Geo<N>::ROW_THREADS);
+    if (sign > 0) hipLaunchKernelGGL((k_fft_rows<N, +1>), grid, block, 0, c->stream, b);
+    else hipLaunchKernelGGL((k_fft_rows<N, -1>), grid, block, 0, c->stream, b);
+    SPR_HIP(c, hipGetLastError());
+    return SPR_OK;
+}
+template <int N> int launch_fft_cols(spr_ctx* c, int sign, FieldBatch b, int items) {
+    const dim3 grid((unsigned)((size_t)items * (N / Geo<N>::C))), block(Geo<N>::COL_THREADS);
+    if (sign > 0) hipLaunchKernelGGL((k_fft_cols<N, +1>), grid, block, 0, c->stream, b);
+    else hipLaunchKernelGGL((k_fft_cols<N, -1>), grid, block, 0, c->stream, b);
+    SPR_HIP(c, hipGetLastError());
+    return SPR_OK;
+}
+template <int N> int launch_corr_spec(spr_ctx* c, CorrSpecArgs a, int items) {
+    a.amp = c->amp;
+    a.tw = c->tw;
+    hipLaunchKernelGGL((k_corr_spec_cols<N>), dim3((unsigned)((size_t)items * (N / Geo<N>::C))),
+                       dim3(Geo<N>::COL_THREADS), 0, c->stream, a);
+    SPR_HIP(c, hipGetLastError());
+    return SPR_OK;
+}
+template <int N> int launch_reshift(spr_ctx* c, ReshiftArgs a, int items) {
+    hipLaunchKernelGGL((k_reshift<N>), dim3((N * N + 255) / 256, items), dim3(256), 0, c->stream, a);
+    SPR_HIP(c, hipGetLastError());
+    return SPR_OK;
+}
+template <int N> int launch_cross(spr_ctx* c, CrossArgs a, int items) {
+    hipLaunchKernelGGL((k_cross<N>), dim3((N * N + 255) / 256, items), dim3(256), 0, c->stream, a);
+    SPR_HIP(c, hipGetLastError());
+    return SPR_OK;
+}
+int fft_rows(spr_ctx* c, int sign, FieldBatch b, int items) { SPR_DISPATCH_N(c, launch_fft_rows, c, sign, b, items); }
+int fft_cols(spr_ctx* c, int sign, FieldBatch b, int items) { SPR_DISPATCH_N(c, launch_fft_cols, c, sign, b, items); }
+int corr_spec(spr_ctx* c, const CorrSpecArgs& a, int items) { SPR_DISPATCH_N(c, launch_corr_spec, c, a, items); }
+int reshift_fields(spr_ctx* c, const ReshiftArgs& a, int items) { SPR_DISPATCH_N(c, launch_reshift, c, a, items); }
+int cross_fields(spr_ctx* c, const CrossArgs& a, int items) { SPR_DISPATCH_N(c, launch_cross, c, a, items); }
+
+// 2-D FT (sign +1, unnormalised) or IFT (sign -1, divided by N^2) of listed fields.
+int fft2_fields(spr_ctx* c, int sign, float2* base, const int* index, int items) {
+    if (items <= 0) return SPR_OK;
+    FieldBatch b{base, index, c->tw, 1.f};
+    if (sign > 0) {
+        SPR_TRY2(fft_rows(c, +1, b, items));
+        return fft_cols(c, +1, b, items);
+    }
+    SPR_TRY2(fft_cols(c, -1, b, items));
+    b.scale = 1.0f / (float)((double)c->n * c->n);
+    return fft_rows(c, -1, b, items);
+}
+
+// FT(|src field|) of listed fields into consecutive spectra at dst.
+int modulus_spectra(spr_ctx* c, const float2* src, const int* src_index, float2* dst, int items) {
+    if (items <= 0) return SPR_OK;
+    AuxRowArgs a = aux_row_args(c);
+    a.src = src;
+    a.src_index = src_index;
+    a.work = dst;
+    SPR_TRY2(row_aux(c, AROW_ABS_FWD_RHO, a, items));
+    FieldBatch b{dst, nullptr, c->tw, 1.f};
+    return fft_cols(c, +1, b, items);
+}
+
+// Correlate spectra d_index[i] against r_index[i], pick the shifts and write
+// them to shiftp (first index of the strict maximum, direct vs twin).
+int correlate_and_pick(spr_ctx* c, const float2* spec, const int* d_index, const int* r_index, float minr,
+                       float maxr, int unit_scale, const float* pdirect, const float* pflip, ShiftParam* shiftp,
+                       int items) {
+    CorrSpecArgs ca{};
+    ca.spec = spec;
+    ca.d_index = d_index;
+    ca.r_index = r_index;
+    ca.out1 = c->work;
+    ca.out2 = c->work2;
+    ca.win = make_corr_win(c->n, minr, maxr);
+    ca.unit_scale = unit_scale;
+    SPR_TRY2(corr_spec(c, ca, items));
+    SPR_TRY2(fill_u64(c, c->keys, 0ull, 2 * c->cap));
+    AuxRowArgs a = aux_row_args(c);
+    SPR_TRY2(row_aux(c, AROW_CORR, a, items));
+    hipLaunchKernelGGL(k_pick_shift, dim3((items + 63) / 64), dim3(64), 0, c->stream,
+                       (const unsigned long long*)c->keys, (const unsigned long long*)(c->keys + c->cap), pdirect,
+                       pflip, c->n, shiftp, items);
+    SPR_HIP(c, hipGetLastError());
+    return SPR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Masked Fourier loss and gradient (spr_grad.hpp)
+// ---------------------------------------------------------------------------
+template <int N> int launch_col_loss(spr_ctx* c, LossArgs a, int items) {
+    hipLaunchKernelGGL((k_col_loss<N>), dim3((unsigned)((size_t)items * (N / Geo<N>::C))), dim3(Geo<N>::COL_THREADS),
+                       0, c->stream, a);
+    SPR_HIP(c, hipGetLastError());
+    return SPR_OK;
+}
+int col_loss(spr_ctx* c, const LossArgs& a, int items) { SPR_DISPATCH_N(c, launch_col_loss, c, a, items); }
+
+// Runs the handle's launches on another stream for the lifetime of the guard.
+struct StreamSwap {
+    spr_ctx* c;
+    hipStream_t saved;
+    StreamSwap(spr_ctx* c_, hipStream_t s) : c(c_), saved(c_->stream) {
+        if (s) c->stream = s;
+    }
+    ~StreamSwap() { c->stream = saved; }
+};
+
+// grad <- 2 FT^H[f'(|F|^2) F], F = FT(rho), for P fields at device addresses;
+// per-field losses into sums[p*SUMS_PER_ITEM + 7].
+int queue_loss_grad(spr_ctx* c, int P, const float2* rho, float2* grad, int kind, float eps) {
+    const int n = c->n;
+    FieldBatch fwd{c->work, nullptr, c->tw, 1.f, rho};
+    SPR_TRY2(fft_rows(c, +1, fwd, P));
+    LossArgs la{c->work, c->amp, c->tw, c->partial, eps, kind};
+    SPR_TRY2(col_loss(c, la, P));
+    SPR_TRY2(reduce(c, n / cols_per_wg(n), 1, 7, P));
+    FieldBatch adj{grad, nullptr, c->tw, 2.f, c->work};
+    return fft_rows(c, -1, adj, P);
+}
+
+int check_indices(spr_ctx* c, const int32_t* idx, int count, int limit, const char* what) {
+    for (int i = 0; i < count; ++i)
+        if (idx[i] < 0 || idx[i] >= limit) return fail(c, SPR_ERR_ARG, std::string(what) + ": index out of range");
     return SPR_OK;
 }
 
@@ -233,6 +606,24 @@ int spr_create(spr_ctx** out, int n, int capacity, const float* amp, int device)
     if ((e = hipMalloc(&c->supp, np * capacity)) != hipSuccess) return cleanup_fail("hipMalloc supp", e);
     if ((e = hipMalloc(&c->amp, np * sizeof(float))) != hipSuccess) return cleanup_fail("hipMalloc amp", e);
     if ((e = hipMalloc(&c->tw, n * sizeof(float2))) != hipSuccess) return cleanup_fail("hipMalloc tw", e);
+    if ((e = hipMalloc(&c->work2, np * capacity * sizeof(float2))) != hipSuccess)
+        return cleanup_fail("hipMalloc work2", e);
+    {
+        const int per = std::max(n / rows_per_wg(n), n / cols_per_wg(n));
+        c->partial_cap = (size_t)capacity * per * 2;
+        if ((e = hipMalloc(&c->partial, c->partial_cap * sizeof(double))) != hipSuccess)
+            return cleanup_fail("hipMalloc partial", e);
+    }
+    if ((e = hipMalloc(&c->sums, (size_t)capacity * SUMS_PER_ITEM * sizeof(double))) != hipSuccess)
+        return cleanup_fail("hipMalloc sums", e);
+    if ((e = hipMalloc(&c->maxbits, (size_t)capacity * 2 * sizeof(unsigned))) != hipSuccess)
+        return cleanup_fail("hipMalloc maxbits", e);
+    if ((e = hipMalloc(&c->keys, (size_t)capacity * 2 * sizeof(unsigned long long))) != hipSuccess)
+        return cleanup_fail("hipMalloc keys", e);
+    if ((e = hipMalloc(&c->radix, (size_t)capacity * 3 * sizeof(unsigned))) != hipSuccess)
+        return cleanup_fail("hipMalloc radix", e);
+    if ((e = hipMalloc(&c->hist, (size_t)capacity * RADIX_BINS * sizeof(unsigned))) != hipSuccess)
+        return cleanup_fail("hipMalloc hist", e);
     std::vector<float2> tw(n);
     for (int k = 0; k < n; ++k) {
         const double ang = -2.0 * M_PI * (double)k / (double)n;
@@ -257,6 +648,16 @@ int spr_destroy(spr_ctx* c) {
     (void)hipFree(c->supp);
     (void)hipFree(c->amp);
     (void)hipFree(c->tw);
+    (void)hipFree(c->work2);
+    (void)hipFree(c->partial);
+    (void)hipFree(c->sums);
+    (void)hipFree(c->maxbits);
+    (void)hipFree(c->keys);
+    (void)hipFree(c->radix);
+    (void)hipFree(c->hist);
+    for (spr_ctx::Buf* b : {&c->spec, &c->xf, &c->xs, &c->supp2, &c->fieldbuf, &c->ampavg, &c->xidx, &c->shiftp,
+                            &c->xchild, &c->xmaps, &c->xphase, &c->adam_m, &c->adam_v})
+        (void)hipFree(b->p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SPR_OK;
@@ -356,6 +757,461 @@ int spr_iterate(spr_ctx* c, int alg, int niter, float beta, float reality) {
     SPR_TRY(queue_iterate(c, alg, niter, beta, reality));
     SPR_HIP(c, hipStreamSynchronize(c->stream));
     return harvest_timing(c);
+}
+
+int spr_sw(spr_ctx* c, float sigma, float threshold, float reality) {
+    (void)reality;
+    if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
+    SPR_HIP(c, hipSetDevice(c->device));
+    SPR_TRY(queue_sw(c, sigma, threshold));
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    return harvest_timing(c);
+}
+
+int spr_swa(spr_ctx* c, float sigma, int spixels, float reality) {
+    (void)reality;
+    if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
+    SPR_HIP(c, hipSetDevice(c->device));
+    SPR_TRY(queue_swa(c, sigma, spixels));
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    return harvest_timing(c);
+}
+
+int spr_evaluate(spr_ctx* c, float reality, float* error_fourier, float* error_real, float* complexsum,
+                 float* modulesum, float* symmetry) {
+    if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
+    if (c->P <= 0) return SPR_OK;
+    SPR_HIP(c, hipSetDevice(c->device));
+    SPR_TRY(queue_error(c, reality));
+    std::vector<double> sums((size_t)c->P * SUMS_PER_ITEM);
+    std::vector<unsigned> mx(c->P);
+    SPR_HIP(c, hipMemcpyAsync(sums.data(), c->sums, sums.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SPR_HIP(c, hipMemcpyAsync(mx.data(), c->maxbits, mx.size() * sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    std::vector<unsigned long long> keys(2 * (size_t)c->cap);
+    if (symmetry) {
+        SPR_TRY(queue_symmetry(c, 0.f, 1.f));
+        SPR_HIP(c, hipMemcpyAsync(keys.data(), c->keys, keys.size() * sizeof(unsigned long long),
+                                  hipMemcpyDeviceToHost, c->stream));
+    }
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    for (int p = 0; p < c->P; ++p) {
+        const double* s = &sums[(size_t)p * SUMS_PER_ITEM];
+        float fm;
+        std::memcpy(&fm, &mx[p], 4);
+        if (error_fourier) error_fourier[p] = (float)std::sqrt(s[1] / s[2]);
+        if (error_real) error_real[p] = (float)std::sqrt(s[3] / s[4]);
+        if (complexsum) complexsum[p] = fm;
+        if (modulesum) modulesum[p] = (float)s[0];
+        if (symmetry) {
+            const float m1 = key_value(keys[p]), m2 = key_value(keys[c->cap + p]);
+            symmetry[p] = std::sqrt(m2 / m1);
+        }
+    }
+    return harvest_timing(c);
+}
+
+int spr_symmetry(spr_ctx* c, float symmin, float symmax, float* symmetry) {
+    if (!c || !symmetry) return fail(c, SPR_ERR_ARG, "NULL argument");
+    if (c->P <= 0) return SPR_OK;
+    SPR_HIP(c, hipSetDevice(c->device));
+    SPR_TRY(queue_symmetry(c, symmin, symmax));
+    std::vector<unsigned long long> keys(2 * (size_t)c->cap);
+    SPR_HIP(c, hipMemcpyAsync(keys.data(), c->keys, keys.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              c->stream));
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    for (int p = 0; p < c->P; ++p) {
+        const float m1 = key_value(keys[p]), m2 = key_value(keys[c->cap + p]);
+        symmetry[p] = std::sqrt(m2 / m1);
+    }
+    return harvest_timing(c);
+}
+
+int spr_run(spr_ctx* c, const spr_op* ops, int nops, int nrep, float reality) {
+    if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
+    if (nops < 0 || nrep < 0 || (nops > 0 && !ops)) return fail(c, SPR_ERR_ARG, "bad op list");
+    for (int i = 0; i < nops; ++i) {
+        const spr_op& o = ops[i];
+        if (o.kind < SPR_OP_ER || o.kind > SPR_OP_GRAD) return fail(c, SPR_ERR_ARG, "unknown op kind");
+        if (o.kind <= SPR_OP_RAAR && o.iterations < 0) return fail(c, SPR_ERR_ARG, "negative iteration count");
+    }
+    SPR_HIP(c, hipSetDevice(c->device));
+    for (int r = 0; r < nrep; ++r) {
+        for (int i = 0; i < nops; ++i) {
+            const spr_op& o = ops[i];
+            switch (o.kind) {
+                case SPR_OP_ER: SPR_TRY(queue_iterate(c, SPR_ER, o.iterations, 0.f, reality)); break;
+                case SPR_OP_HIO: SPR_TRY(queue_iterate(c, SPR_HIO, o.iterations, o.beta, reality)); break;
+                case SPR_OP_RAAR: SPR_TRY(queue_iterate(c, SPR_RAAR, o.iterations, o.beta, reality)); break;
+                case SPR_OP_SW: SPR_TRY(queue_sw(c, o.sigma, o.threshold)); break;
+                case SPR_OP_SWA: SPR_TRY(queue_swa(c, o.sigma, o.pixels)); break;
+                case SPR_OP_GRAD: SPR_TRY(spr_refine(c, o.iterations, o.beta, o.pixels, o.threshold, nullptr)); break;
+            }
+        }
+    }
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    return harvest_timing(c);
+}
+
+int spr_align(spr_ctx* c, const float* ref_rho, float minr, float maxr, const float* phase_direct,
+              const float* phase_flip, int32_t* shifts) {
+    if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
+    if (!ref_rho || !phase_direct || !phase_flip) return fail(c, SPR_ERR_ARG, "NULL argument");
+    const int P = c->P;
+    if (P <= 0) return SPR_OK;
+    const size_t np = (size_t)c->n * c->n;
+    SPR_HIP(c, hipSetDevice(c->device));
+    SPR_TRY(ensure(c, c->spec, (P + 1) * np * sizeof(float2)));
+    SPR_TRY(ensure(c, c->fieldbuf, np * sizeof(float2)));
+    SPR_TRY(ensure(c, c->supp2, (size_t)c->cap * np));
+    SPR_TRY(ensure(c, c->shiftp, (size_t)P * sizeof(ShiftParam)));
+    SPR_TRY(ensure(c, c->xphase, 2 * (size_t)P * sizeof(float)));
+    SPR_TRY(ensure(c, c->xidx, 2 * (size_t)P * sizeof(int)));
+    std::vector<int> idx(2 * (size_t)P);
+    for (int i = 0; i < P; ++i) {
+        idx[i] = i;        // d: the individual's spectrum
+        idx[P + i] = P;    // r: the reference spectrum
+    }
+    float2* spec = (float2*)c->spec.p;
+    float* ph = (float*)c->xphase.p;
+    int* di = (int*)c->xidx.p;
+    ShiftParam* sp = (ShiftParam*)c->shiftp.p;
+    SPR_HIP(c, hipMemcpyAsync(c->fieldbuf.p, ref_rho, np * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+    SPR_HIP(c, hipMemcpyAsync(ph, phase_direct, P * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    SPR_HIP(c, hipMemcpyAsync(ph + P, phase_flip, P * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    SPR_HIP(c, hipMemcpyAsync(di, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    // set_ref_density and get_shift both unit-scale their spectra (normalize_ft_shift)
+    SPR_TRY(modulus_spectra(c, c->rho, nullptr, spec, P));
+    SPR_TRY(modulus_spectra(c, (const float2*)c->fieldbuf.p, nullptr, spec + (size_t)P * np, 1));
+    SPR_TRY(correlate_and_pick(c, spec, di, di + P, minr, maxr, 1, ph, ph + P, sp, P));
+    ReshiftArgs ra{};
+    ra.src = c->rho;
+    ra.ssupp = c->supp;
+    ra.dst = c->work;
+    ra.dsupp = (uint8_t*)c->supp2.p;
+    ra.shift = sp;
+    SPR_TRY(reshift_fields(c, ra, P));
+    SPR_HIP(c, hipMemcpyAsync(c->rho, c->work, P * np * sizeof(float2), hipMemcpyDeviceToDevice, c->stream));
+    SPR_HIP(c, hipMemcpyAsync(c->supp, c->supp2.p, P * np, hipMemcpyDeviceToDevice, c->stream));
+    std::vector<ShiftParam> hs(P);
+    SPR_HIP(c, hipMemcpyAsync(hs.data(), sp, P * sizeof(ShiftParam), hipMemcpyDeviceToHost, c->stream));
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    if (shifts)
+        for (int i = 0; i < P; ++i) {
+            shifts[3 * i] = hs[i].sx;
+            shifts[3 * i + 1] = hs[i].sy;
+            shifts[3 * i + 2] = hs[i].flip;
+        }
+    return harvest_timing(c);
+}
+
+int spr_average(spr_ctx* c, const int32_t* idx, int navg, float* rho_out, int32_t* supp_out) {
+    if (!c || !idx || !rho_out || !supp_out) return fail(c, SPR_ERR_ARG, "NULL argument");
+    if (navg < 1) return fail(c, SPR_ERR_ARG, "navg must be >= 1");
+    SPR_TRY(check_indices(c, idx, navg, c->P, "spr_average"));
+    const size_t np = (size_t)c->n * c->n;
+    SPR_HIP(c, hipSetDevice(c->device));
+    SPR_TRY(ensure(c, c->xidx, navg * sizeof(int)));
+    SPR_TRY(ensure(c, c->fieldbuf, np * sizeof(float2)));
+    SPR_TRY(ensure(c, c->supp2, (size_t)c->cap * np));
+    SPR_HIP(c, hipMemcpyAsync(c->xidx.p, idx, navg * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    const float norm = (float)(1. / (float)navg);
+    hipLaunchKernelGGL(k_average, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, c->stream, (const float2*)c->rho,
+                       (const uint8_t*)c->supp, (const int*)c->xidx.p, navg, (int)np, norm, (float2*)c->fieldbuf.p,
+                       (uint8_t*)c->supp2.p);
+    SPR_HIP(c, hipGetLastError());
+    std::vector<uint8_t> s(np);
+    SPR_HIP(c, hipMemcpyAsync(rho_out, c->fieldbuf.p, np * sizeof(float2), hipMemcpyDeviceToHost, c->stream));
+    SPR_HIP(c, hipMemcpyAsync(s.data(), c->supp2.p, np, hipMemcpyDeviceToHost, c->stream));
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < np; ++i) supp_out[i] = s[i];
+    return harvest_timing(c);
+}
+
+int spr_average_module(spr_ctx* c, const int32_t* idx, int navg, int ft, float* out) {
+    if (!c || !idx || !out) return fail(c, SPR_ERR_ARG, "NULL argument");
+    if (navg < 1) return fail(c, SPR_ERR_ARG, "navg must be >= 1");
+    SPR_TRY(check_indices(c, idx, navg, c->P, "spr_average_module"));
+    const size_t np = (size_t)c->n * c->n;
+    SPR_HIP(c, hipSetDevice(c->device));
+    SPR_TRY(ensure(c, c->xidx, navg * sizeof(int)));
+    SPR_TRY(ensure(c, c->ampavg, np * sizeof(float)));
+    SPR_HIP(c, hipMemcpyAsync(c->xidx.p, idx, navg * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    const float norm = (float)(1. / (float)navg);
+    const dim3 g1((unsigned)((np + 255) / 256));
+    if (ft) {
+        SPR_TRY(ensure(c, c->spec, (size_t)navg * np * sizeof(float2)));
+        hipLaunchKernelGGL(k_copy_fields, dim3((unsigned)((np + 255) / 256), navg), dim3(256), 0, c->stream,
+                           (const float2*)c->rho, (const int*)c->xidx.p, (float2*)c->spec.p, (const int*)nullptr,
+                           (int)np);
+        SPR_HIP(c, hipGetLastError());
+        SPR_TRY(fft2_fields(c, +1, (float2*)c->spec.p, nullptr, navg));
+        hipLaunchKernelGGL(k_sum_modulus, g1, dim3(256), 0, c->stream, (const float2*)c->spec.p, (const int*)nullptr,
+                           navg, (int)np, norm, (float*)c->ampavg.p);
+    } else {
+        hipLaunchKernelGGL(k_sum_modulus, g1, dim3(256), 0, c->stream, (const float2*)c->rho, (const int*)c->xidx.p,
+                           navg, (int)np, norm, (float*)c->ampavg.p);
+    }
+    SPR_HIP(c, hipGetLastError());
+    SPR_HIP(c, hipMemcpyAsync(out, c->ampavg.p, np * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    return harvest_timing(c);
+}
+
+int spr_crossover(spr_ctx* c, int nchild, const spr_cross_desc* desc, const int32_t* crossmaps, int crossmap_len,
+                  const float* avg_rho, const float* amp_avg, float crossdiff, float use_average, float minr,
+                  float maxr, float reality, int update_support, float* child_rho, int32_t* child_supp) {
+    (void)update_support;
+    if (!c || !desc || !crossmaps || !avg_rho || !amp_avg || !child_rho || !child_supp)
+        return fail(c, SPR_ERR_ARG, "NULL argument");
+    const int P = c->P, K = nchild, n = c->n;
+    if (K < 1 || K > c->cap) return fail(c, SPR_ERR_ARG, "number of children exceeds the capacity");
+    const size_t np = (size_t)n * n;
+    for (int k = 0; k < K; ++k) {
+        const spr_cross_desc& d = desc[k];
+        const int pp[4] = {d.p0, d.p1, d.p2, d.p3};
+        SPR_TRY(check_indices(c, pp, 4, P, "spr_crossover parent"));
+        if (d.squaresizex < 1 || d.squaresizey < 1) return fail(c, SPR_ERR_ARG, "cell size must be >= 1");
+        if (d.crosssizex != (n + d.squaresizex - 1) / d.squaresizex + 1 ||
+            d.crosssizey != (n + d.squaresizey - 1) / d.squaresizey + 1)
+            return fail(c, SPR_ERR_ARG, "crossmap grid does not match the cell sizes");
+        if (d.offsetx < 0 || d.offsetx >= d.squaresizex + 1 || d.offsety < 0 || d.offsety >= d.squaresizey + 1)
+            return fail(c, SPR_ERR_ARG, "cell offset out of range");
+        if (d.crossmap_offset < 0 || d.crossmap_offset + d.crosssizex * d.crosssizey > crossmap_len)
+            return fail(c, SPR_ERR_ARG, "crossmap out of range");
+    }
+    SPR_HIP(c, hipSetDevice(c->device));
+    const bool with_avg = use_average != 0.f;
+    // fields: child k -> 5k+0 p0, 5k+1 p1, 5k+2 p2, 5k+3 p3, 5k+4 avg; supports: 3k+0 p0, 3k+1 p2, 3k+2 p3
+    SPR_TRY(ensure(c, c->xf, 5 * (size_t)K * np * sizeof(float2)));
+    SPR_TRY(ensure(c, c->xs, 3 * (size_t)K * np));
+    SPR_TRY(ensure(c, c->spec, (size_t)(P + 1) * np * sizeof(float2)));
+    SPR_TRY(ensure(c, c->fieldbuf, np * sizeof(float2)));
+    SPR_TRY(ensure(c, c->ampavg, np * sizeof(float)));
+    SPR_TRY(ensure(c, c->shiftp, (size_t)K * sizeof(ShiftParam)));
+    SPR_TRY(ensure(c, c->xchild, (size_t)K * sizeof(CrossChild)));
+    SPR_TRY(ensure(c, c->xmaps, (size_t)std::max(crossmap_len, 1) * sizeof(int32_t)));
+    SPR_TRY(ensure(c, c->xphase, 8 * (size_t)K * sizeof(float)));
+    // host-side index lists, uploaded once
+    enum { I_D = 0, I_R, I_SRC, I_DST, I_SSRC, I_SDST, NLIST };
+    const int L = 4 * K;   // one list slot per (role, child)
+    std::vector<int> lists((size_t)NLIST * L + 8 * (size_t)K, 0);
+    auto list = [&](int which, int role, int k) -> int& { return lists[(size_t)which * L + role * K + k]; };
+    std::vector<float> phases(8 * (size_t)K);
+    std::vector<CrossChild> cc(K);
+    int* ftl = &lists[(size_t)NLIST * L];            // fields to transform (crossft children), up to 5K
+    int nft = 0;
+    std::vector<int> child_fields, er_f, er_s;
+    for (int k = 0; k < K; ++k) {
+        const spr_cross_desc& d = desc[k];
+        const int roles[4] = {d.p0, d.p2, d.p3, P};   // spectrum / source of each role (P = the average)
+        for (int r = 0; r < 4; ++r) {
+            list(I_D, r, k) = roles[r];
+            list(I_R, r, k) = d.p1;
+            list(I_SRC, r, k) = r < 3 ? roles[r] : 0;
+            list(I_DST, r, k) = 5 * k + (r == 0 ? 0 : r == 1 ? 2 : r == 2 ? 3 : 4);
+            list(I_SSRC, r, k) = r < 3 ? roles[r] : 0;
+            list(I_SDST, r, k) = 3 * k + (r < 3 ? r : 0);
+            phases[(size_t)r * 2 * K + k] = d.phase[2 * r];
+            phases[(size_t)r * 2 * K + K + k] = d.phase[2 * r + 1];
+        }
+        CrossChild& x = cc[k];
+        x.crossft = d.crossft;
+        x.crosssym = d.crosssym;
+        x.crosssizex = d.crosssizex;
+        x.squaresizex = d.squaresizex;
+        x.squaresizey = d.squaresizey;
+        x.offsetx = d.offsetx;
+        x.offsety = d.offsety;
+        x.rotangle = d.rotangle;
+        x.crossmap_offset = d.crossmap_offset;
+        x.f0 = 5 * k;
+        x.f1 = 5 * k + 1;
+        x.f2 = 5 * k + 2;
+        x.f3 = 5 * k + 3;
+        x.favg = 5 * k + 4;
+        x.s0 = 3 * k;
+        x.s1 = d.p1;
+        x.s2 = 3 * k + 1;
+        x.s3 = 3 * k + 2;
+        if (d.crossft) {
+            for (int f = 0; f < (with_avg ? 5 : 4); ++f) ftl[nft++] = 5 * k + f;
+            child_fields.push_back(5 * k);
+        }
+        for (int r = 0; r < 3; ++r) {
+            er_f.push_back(5 * k + (r == 0 ? 0 : r + 1));
+            er_s.push_back(3 * k + r);
+        }
+    }
+    // p1 copies: field 5k+1 <- rho[p1]; and the ER lists of p1 with the population supports
+    std::vector<int> p1_src(K), p1_dst(K);
+    for (int k = 0; k < K; ++k) {
+        p1_src[k] = desc[k].p1;
+        p1_dst[k] = 5 * k + 1;
+    }
+    const int nchild_ft = (int)child_fields.size();
+    std::vector<int> out_f(K), out_s(K);   // every child: its field and support slots
+    for (int k = 0; k < K; ++k) {
+        out_f[k] = 5 * k;
+        out_s[k] = 3 * k;
+    }
+    std::vector<int> tail;
+    tail.insert(tail.end(), p1_src.begin(), p1_src.end());
+    tail.insert(tail.end(), p1_dst.begin(), p1_dst.end());
+    tail.insert(tail.end(), er_f.begin(), er_f.end());
+    tail.insert(tail.end(), er_s.begin(), er_s.end());
+    tail.insert(tail.end(), child_fields.begin(), child_fields.end());
+    tail.insert(tail.end(), out_f.begin(), out_f.end());
+    tail.insert(tail.end(), out_s.begin(), out_s.end());
+    const size_t tail_off = lists.size();
+    lists.insert(lists.end(), tail.begin(), tail.end());
+    SPR_TRY(ensure(c, c->xidx, lists.size() * sizeof(int)));
+    int* dl = (int*)c->xidx.p;
+    auto dlist = [&](int which, int role) { return dl + (size_t)which * L + role * K; };
+    int* d_ft = dl + (size_t)NLIST * L;
+    int* d_p1src = dl + tail_off;
+    int* d_p1dst = d_p1src + K;
+    int* d_erf = d_p1dst + K;
+    int* d_ers = d_erf + 3 * K;
+    int* d_child = d_ers + 3 * K;
+    int* d_outf = d_child + nchild_ft;
+    int* d_outs = d_outf + K;
+    float* dph = (float*)c->xphase.p;
+    float2* xf = (float2*)c->xf.p;
+    uint8_t* xs = (uint8_t*)c->xs.p;
+    float2* spec = (float2*)c->spec.p;
+    ShiftParam* sp = (ShiftParam*)c->shiftp.p;
+    SPR_HIP(c, hipMemcpyAsync(dl, lists.data(), lists.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    SPR_HIP(c, hipMemcpyAsync(dph, phases.data(), phases.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    SPR_HIP(c, hipMemcpyAsync(c->xchild.p, cc.data(), K * sizeof(CrossChild), hipMemcpyHostToDevice, c->stream));
+    SPR_HIP(c, hipMemcpyAsync(c->xmaps.p, crossmaps, (size_t)crossmap_len * sizeof(int32_t), hipMemcpyHostToDevice,
+                              c->stream));
+    SPR_HIP(c, hipMemcpyAsync(c->fieldbuf.p, avg_rho, np * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+    SPR_HIP(c, hipMemcpyAsync(c->ampavg.p, amp_avg, np * sizeof(float), hipMemcpyHostToDevice, c->stream));
+
+    // prepare_parents: spectra FT(|rho|) of the population and of the average (not unit-scaled)
+    SPR_TRY(modulus_spectra(c, c->rho, nullptr, spec, P));
+    if (with_avg) SPR_TRY(modulus_spectra(c, (const float2*)c->fieldbuf.p, nullptr, spec + (size_t)P * np, 1));
+    // reshift_parent of p0, p2, p3 (and the average) against p1
+    for (int r = 0; r < (with_avg ? 4 : 3); ++r) {
+        SPR_TRY(correlate_and_pick(c, spec, dlist(I_D, r), dlist(I_R, r), minr, maxr, 0, dph + (size_t)r * 2 * K,
+                                   dph + (size_t)r * 2 * K + K, sp, K));
+        ReshiftArgs ra{};
+        ra.src = r < 3 ? c->rho : (const float2*)c->fieldbuf.p;
+        ra.src_index = dlist(I_SRC, r);
+        ra.ssupp = r < 3 ? c->supp : nullptr;
+        ra.ssupp_index = dlist(I_SSRC, r);
+        ra.dst = xf;
+        ra.dst_index = dlist(I_DST, r);
+        ra.dsupp = r < 3 ? xs : nullptr;
+        ra.dsupp_index = dlist(I_SDST, r);
+        ra.shift = sp;
+        SPR_TRY(reshift_fields(c, ra, K));
+    }
+    // get_new_individual: p1 copy, ER projections, transforms, cell mix, inverse, ER
+    const dim3 gp((unsigned)((np + 255) / 256), K), gp3((unsigned)((np + 255) / 256), 3 * K);
+    hipLaunchKernelGGL(k_copy_fields, gp, dim3(256), 0, c->stream, (const float2*)c->rho, (const int*)d_p1src, xf,
+                       (const int*)d_p1dst, (int)np);
+    SPR_HIP(c, hipGetLastError());
+    const PhaseCut pc = cut_er_hio(reality);
+    hipLaunchKernelGGL(k_er_project, gp, dim3(256), 0, c->stream, xf, (const int*)d_p1dst, (const uint8_t*)c->supp,
+                       (const int*)d_p1src, (int)np, pc);
+    SPR_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(k_er_project, gp3, dim3(256), 0, c->stream, xf, (const int*)d_erf, (const uint8_t*)xs,
+                       (const int*)d_ers, (int)np, pc);
+    SPR_HIP(c, hipGetLastError());
+    SPR_TRY(fft2_fields(c, +1, xf, d_ft, nft));
+    CrossArgs xa{};
+    xa.fields = xf;
+    xa.supps = xs;
+    xa.pop_supp = c->supp;
+    xa.child = (const CrossChild*)c->xchild.p;
+    xa.crossmaps = (const int32_t*)c->xmaps.p;
+    xa.amp = c->amp;
+    xa.amp_avg = (const float*)c->ampavg.p;
+    xa.crossdiff = crossdiff;
+    xa.use_average = use_average;
+    SPR_TRY(cross_fields(c, xa, K));
+    SPR_TRY(fft2_fields(c, -1, xf, d_child, nchild_ft));
+    hipLaunchKernelGGL(k_er_project, gp, dim3(256), 0, c->stream, xf, (const int*)d_outf, (const uint8_t*)xs,
+                       (const int*)d_outs, (int)np, pc);
+    SPR_HIP(c, hipGetLastError());
+    for (int k = 0; k < K; ++k) {
+        SPR_HIP(c, hipMemcpyAsync(child_rho + 2 * (size_t)k * np, xf + (size_t)(5 * k) * np, np * sizeof(float2),
+                                  hipMemcpyDeviceToHost, c->stream));
+    }
+    std::vector<uint8_t> hs(3 * (size_t)K * np);
+    SPR_HIP(c, hipMemcpyAsync(hs.data(), xs, hs.size(), hipMemcpyDeviceToHost, c->stream));
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < K; ++k)
+        for (size_t i = 0; i < np; ++i) child_supp[(size_t)k * np + i] = hs[(size_t)(3 * k) * np + i];
+    return harvest_timing(c);
+}
+
+int spr_loss_grad(spr_ctx* c, int P, const float* rho, float* grad, double* loss, int kind, float eps, void* stream) {
+    if (!c || !rho || !grad) return fail(c, SPR_ERR_ARG, "NULL argument");
+    if (P < 1 || P > c->cap) return fail(c, SPR_ERR_ARG, "field count exceeds the capacity");
+    if (kind != LOSS_POISSON && kind != LOSS_AMPLITUDE) return fail(c, SPR_ERR_ARG, "unknown loss kind");
+    SPR_HIP(c, hipSetDevice(c->device));
+    StreamSwap guard(c, (hipStream_t)stream);
+    SPR_TRY(queue_loss_grad(c, P, (const float2*)rho, (float2*)grad, kind, eps));
+    if (loss) {
+        std::vector<double> s((size_t)P * SUMS_PER_ITEM);
+        SPR_HIP(c, hipMemcpyAsync(s.data(), c->sums, s.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        SPR_HIP(c, hipStreamSynchronize(c->stream));
+        for (int p = 0; p < P; ++p) loss[p] = s[(size_t)p * SUMS_PER_ITEM + 7];
+    }
+    return harvest_timing(c);
+}
+
+int spr_refine(spr_ctx* c, int steps, float lr, int kind, float eps, double* loss) {
+    if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
+    if (steps < 0) return fail(c, SPR_ERR_ARG, "negative step count");
+    if (kind != LOSS_POISSON && kind != LOSS_AMPLITUDE) return fail(c, SPR_ERR_ARG, "unknown loss kind");
+    const int P = c->P;
+    if (P <= 0 || steps == 0) return SPR_OK;
+    const size_t n = (size_t)c->n * c->n * P;
+    SPR_HIP(c, hipSetDevice(c->device));
+    SPR_TRY(ensure(c, c->adam_m, n * sizeof(float2)));
+    SPR_TRY(ensure(c, c->adam_v, n * sizeof(float2)));
+    SPR_TRY(fill_u32(c, (unsigned*)c->adam_m.p, 0u, (int)(2 * n)));
+    SPR_TRY(fill_u32(c, (unsigned*)c->adam_v.p, 0u, (int)(2 * n)));
+    const float b1 = 0.9f, b2 = 0.999f, aeps = 1e-8f;
+    float p1 = 1.f, p2 = 1.f;
+    for (int s = 1; s <= steps; ++s) {
+        SPR_TRY(queue_loss_grad(c, P, c->rho, c->work2, kind, eps));
+        p1 *= b1;
+        p2 *= b2;
+        hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->rho,
+                           (const float2*)c->work2, (float2*)c->adam_m.p, (float2*)c->adam_v.p, n, lr, b1, b2, aeps,
+                           1.f - p1, 1.f - p2);
+        SPR_HIP(c, hipGetLastError());
+    }
+    if (loss) {
+        SPR_TRY(queue_loss_grad(c, P, c->rho, c->work2, kind, eps));
+        std::vector<double> sm((size_t)P * SUMS_PER_ITEM);
+        SPR_HIP(c, hipMemcpyAsync(sm.data(), c->sums, sm.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        SPR_HIP(c, hipStreamSynchronize(c->stream));
+        for (int p = 0; p < P; ++p) loss[p] = sm[(size_t)p * SUMS_PER_ITEM + 7];
+    }
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    return harvest_timing(c);
+}
+
+int spr_device_count(int* n) {
+    if (!n) return fail(nullptr, SPR_ERR_ARG, "NULL argument");
+    *n = 0;
+    int k = 0;
+    if (hipGetDeviceCount(&k) != hipSuccess) return SPR_OK;
+    *n = k;
+    return SPR_OK;
+}
+
+int spr_device_name(int device, char* buf, int len) {
+    if (!buf || len <= 0) return fail(nullptr, SPR_ERR_ARG, "NULL argument");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(nullptr, SPR_ERR_HIP, "bad device");
+    std::snprintf(buf, (size_t)len, "%s", prop.name);
+    return SPR_OK;
 }
 
 int spr_sync(spr_ctx* c) {

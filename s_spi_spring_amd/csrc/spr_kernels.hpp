@@ -34,6 +34,13 @@ template <int N> struct Geo {
     static constexpr int C = (8192 / N) < N ? (8192 / N) : N;  // columns per column-pass workgroup
     static constexpr int COL_THREADS = C * T;
     static constexpr int CS = C + 1;                      // padded LDS row length of a column tile
+    // Two column workgroups per CU when their LDS fits (160 KB): waves per SIMD to target.
+    static constexpr int COL_LDS = (N * CS + N) * 8 + 512;
+#ifndef SPR_COL_WG_PER_CU
+#define SPR_COL_WG_PER_CU 2
+#endif
+    static constexpr int COL_WAVES_PER_EU =
+        (SPR_COL_WG_PER_CU * COL_LDS <= 160 * 1024) ? SPR_COL_WG_PER_CU * COL_THREADS / 256 : COL_THREADS / 256;
     static_assert(RW >= 1 && ROW_THREADS % T == 0, "row geometry");
     static_assert(COL_THREADS <= 1024, "column geometry");
 };
@@ -72,13 +79,22 @@ __device__ __forceinline__ bool phase_fails(float x, float y, const PhaseCut& pc
     return ph >= pc.hi_ge || ph <= pc.lo_le;
 }
 
+// a * (cos phi, sin phi) with phi = atan2f(y, x) for a zero (or NaN) value:
+// atan2f(+-0, +0) = +-0 and atan2f(+-0, -0) = +-pi_f, whose cosine rounds to -1
+// and whose sine is -+8.742278e-8 (sin of the float nearest pi).
+__device__ __forceinline__ float2 polar_of_zero(float2 F, float a) {
+    if (F.x != F.x || F.y != F.y) return make_float2(F.x + F.y, F.x + F.y);
+    if (!signbit(F.x)) return make_float2(a, copysignf(0.f, F.y) * a);
+    const float s = 8.742278e-8f;
+    return make_float2(-a, signbit(F.y) ? s * a : -s * a);
+}
+
 // Fourier modulus projection P_M (sub_intensities).
 __device__ __forceinline__ float2 project_modulus(float2 F, float a) {
     const float m = sqrtf(F.y * F.y + F.x * F.x);
     if (a >= 0.f) {
         if (m > 0.f) return make_float2(a * F.x / m, a * F.y / m);
-        const float ph = atan2f(F.y, F.x);
-        return make_float2(a * cosf(ph), a * sinf(ph));
+        return polar_of_zero(F, a);
     }
     if (a < -1.5f && m > -a - 2.f) {
         const float nm = -a - 2.f;
@@ -122,45 +138,50 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
     const int t = tid % T;
     const int rl = tid / T;
     for (int i = tid; i < N; i += G::ROW_THREADS) tw[i] = a.tw[i];
-    const size_t base = ((size_t)blockIdx.x * G::RW + rl) * N;
+    // Block-uniform bases with 32-bit per-thread offsets (SGPR base addressing).
+    const size_t blk = (size_t)blockIdx.x * G::RW * N;
+    float2* __restrict__ rho = a.rho + blk;
+    float2* __restrict__ work = a.work + blk;
+    const uint8_t* __restrict__ supp = a.supp + blk;
+    const int off = rl * N + t;
     const RowAcc acc{lds + rl * G::RS};
     float2 v[8];
     __syncthreads();
 
     if constexpr (MODE == ROW_FWD || MODE == ROW_FWD_INPLACE) {
 #pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = a.rho[base + t + r * T];
+        for (int r = 0; r < 8; ++r) v[r] = rho[off + r * T];
         fft_line<N, +1>(v, acc, tw, t);
-        float2* dst = (MODE == ROW_FWD) ? a.work : a.rho;
+        float2* dst = (MODE == ROW_FWD) ? work : rho;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) dst[base + t + q * T] = v[q];
+        for (int q = 0; q < 8; ++q) dst[off + q * T] = v[q];
     } else if constexpr (MODE == ROW_FINAL || MODE == ROW_INV_INPLACE) {
-        const float2* src = (MODE == ROW_FINAL) ? a.work : a.rho;
+        const float2* src = (MODE == ROW_FINAL) ? work : rho;
 #pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = src[base + t + r * T];
+        for (int r = 0; r < 8; ++r) v[r] = src[off + r * T];
         fft_line<N, -1>(v, acc, tw, t);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) a.rho[base + t + q * T] = make_float2(v[q].x * a.inv_nn, v[q].y * a.inv_nn);
+        for (int q = 0; q < 8; ++q) rho[off + q * T] = make_float2(v[q].x * a.inv_nn, v[q].y * a.inv_nn);
     } else {
 #pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = a.work[base + t + r * T];
+        for (int r = 0; r < 8; ++r) v[r] = work[off + r * T];
         fft_line<N, -1>(v, acc, tw, t);
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const size_t i = base + t + q * T;
+            const int i = off + q * T;
             const float2 d = make_float2(v[q].x * a.inv_nn, v[q].y * a.inv_nn);  // rho'
-            const uint8_t s = a.supp[i];
+            const uint8_t s = supp[i];
             float2 o;
             if constexpr (MODE == ROW_ER) {
                 const float ts = (s != 0 && !phase_fails(d.x, d.y, a.pc)) ? 1.f : 0.f;
                 o = make_float2(d.x * ts, d.y * ts);
             } else if constexpr (MODE == ROW_HIO) {
-                const float2 b = a.rho[i];
+                const float2 b = rho[i];
                 const float ts = (s != 0 && !phase_fails(d.x, d.y, a.pc)) ? 1.f : 0.f;
                 o = make_float2((b.x - a.beta * d.x) * (1.f - ts) + d.x * ts,
                                 (b.y - a.beta * d.y) * (1.f - ts) + d.y * ts);
             } else {  // RAAR
-                const float2 b = a.rho[i];
+                const float2 b = rho[i];
                 const float qx = 2.f * d.x - b.x, qy = 2.f * d.y - b.y;
                 o = d;
                 if (s == 0 || phase_fails(qx, qy, a.pc)) {
@@ -168,12 +189,12 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
                     o = make_float2(a.beta * b.x + c * d.x, a.beta * b.y + c * d.y);
                 }
             }
-            if constexpr (MODE != ROW_ER) a.rho[i] = o;
+            if constexpr (MODE != ROW_ER) rho[i] = o;
             v[q] = o;
         }
         fft_line<N, +1>(v, acc, tw, t);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) a.work[base + t + q * T] = v[q];
+        for (int q = 0; q < 8; ++q) work[off + q * T] = v[q];
     }
 }
 
@@ -193,7 +214,8 @@ struct ColArgs {
 };
 
 template <int N, int MODE>
-__global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col(ColArgs a) {
+__global__ __launch_bounds__(Geo<N>::COL_THREADS) __attribute__((amdgpu_waves_per_eu(Geo<N>::COL_WAVES_PER_EU)))
+void k_col(ColArgs a) {
     using G = Geo<N>;
     constexpr int T = G::T, C = G::C;
     __shared__ float2 tw[N];
@@ -205,20 +227,23 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col(ColArgs a) {
     constexpr int TILES = N / C;
     const int p = blockIdx.x / TILES;
     const int x0 = (blockIdx.x % TILES) * C;
-    const size_t base = (size_t)p * N * N + x0 + c;
+    // Block-uniform bases with 32-bit per-thread offsets (SGPR base addressing).
+    float2* __restrict__ f = a.f + (size_t)p * N * N + x0;
+    const float* __restrict__ amp = a.amp + x0;
+    const int off = t * N + c;
     const ColAcc<G::CS> acc{lds + c};
     float2 v[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = a.f[base + (size_t)(t + r * T) * N];
+    for (int r = 0; r < 8; ++r) v[r] = f[off + r * T * N];
     __syncthreads();
     if constexpr (MODE == COL_PROJ || MODE == COL_FWD) fft_line<N, +1>(v, acc, tw, t);
     if constexpr (MODE == COL_PROJ) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = project_modulus(v[q], a.amp[(size_t)(t + q * T) * N + x0 + c]);
+        for (int q = 0; q < 8; ++q) v[q] = project_modulus(v[q], amp[off + q * T * N]);
     }
     if constexpr (MODE == COL_PROJ || MODE == COL_INV) fft_line<N, -1>(v, acc, tw, t);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) a.f[base + (size_t)(t + q * T) * N] = v[q];
+    for (int q = 0; q < 8; ++q) f[off + q * T * N] = v[q];
 }
 
 }  // namespace spr

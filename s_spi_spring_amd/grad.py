@@ -1,0 +1,62 @@
+"""Differentiable masked Fourier-magnitude loss for PyTorch-ROCm.
+
+FourierLoss is a torch.autograd.Function whose forward and backward both run the
+hand-written HIP kernels behind spr_loss_grad (include/spring_hip.h): one call
+computes the per-field loss (FP64 sums) and its gradient through the adjoint
+FFT; backward scales the saved gradient by the incoming one. Losses (kind):
+
+  "poisson"    L = sum_{A>=0} |F|^2 - I log(|F|^2 + eps),  I = A^2 (counts)
+  "amplitude"  L = sum_{A>=0} (|F| - A)^2
+
+with F = FT(rho) (unnormalised, exp(+2 pi i k.x/N)) and A the corner-origin
+amplitude of the backend. The gradient follows PyTorch's convention for
+complex tensors (dL/dRe + i dL/dIm), so torch.optim works on rho directly.
+"""
+from __future__ import annotations
+
+import torch
+
+KINDS = {"poisson": 0, "amplitude": 1}
+
+
+class FourierLoss(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, rho, backend, kind="poisson", eps=1e-3):
+        if not rho.is_cuda or rho.dtype != torch.complex64:
+            raise TypeError("rho must be a complex64 tensor on the GPU")
+        rho = rho.contiguous()
+        if rho.dim() == 2:
+            rho = rho.unsqueeze(0)
+        P = rho.shape[0]
+        if tuple(rho.shape[1:]) != (backend.n, backend.n):
+            raise ValueError("field size differs from the backend's pattern")
+        grad = torch.empty_like(rho)
+        stream = torch.cuda.current_stream(rho.device).cuda_stream
+        loss = backend.loss_grad(P, rho.data_ptr(), grad.data_ptr(), KINDS[kind], eps, stream)
+        ctx.save_for_backward(grad)
+        return torch.from_numpy(loss).to(rho.device)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (grad,) = ctx.saved_tensors
+        return grad * grad_out.to(torch.float32)[:, None, None], None, None, None
+
+
+def fourier_loss(rho, backend, kind="poisson", eps=1e-3):
+    """Per-field loss (float64 tensor of length P) of rho [P, N, N] (or [N, N])."""
+    return FourierLoss.apply(rho, backend, kind, eps)
+
+
+def refine(rho, backend, steps=20, lr=1e-2, kind="poisson", eps=1e-3):
+    """Adam refinement of rho (complex64 [P, N, N] on the GPU) on the loss;
+    returns the refined tensor and the loss history (list of float64 arrays)."""
+    x = rho.detach().clone().requires_grad_(True)
+    opt = torch.optim.Adam([x], lr=lr)
+    hist = []
+    for _ in range(steps):
+        opt.zero_grad()
+        loss = fourier_loss(x, backend, kind, eps)
+        loss.sum().backward()
+        opt.step()
+        hist.append(loss.detach().cpu().numpy())
+    return x.detach(), hist

@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libspring_hip.so")
+# SPRING_HIP_LIB selects another build of the same library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("SPRING_HIP_LIB") or os.path.join(_HERE, "lib", "libspring_hip.so")
 
 ER, HIO, RAAR = 0, 1, 2
 ALG = {"ER": ER, "HIO": HIO, "RAAR": RAAR}
@@ -42,7 +43,57 @@ SIGNATURES = {
     "spr_stream": (_I, [_VP, ctypes.POINTER(_VP)]),
     "spr_timing": (_I, [_VP, _I]),
     "spr_timing_read": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]),
+    "spr_sw": (_I, [_VP, _F, _F, _F]),
+    "spr_swa": (_I, [_VP, _F, _I, _F]),
+    "spr_evaluate": (_I, [_VP, _F, _VP, _VP, _VP, _VP, _VP]),
+    "spr_symmetry": (_I, [_VP, _F, _F, _VP]),
+    "spr_run": (_I, [_VP, _VP, _I, _I, _F]),
+    "spr_align": (_I, [_VP, _VP, _F, _F, _VP, _VP, _VP]),
+    "spr_average": (_I, [_VP, _VP, _I, _VP, _VP]),
+    "spr_average_module": (_I, [_VP, _VP, _I, _I, _VP]),
+    "spr_crossover": (_I, [_VP, _I, _VP, _VP, _I, _VP, _VP, _F, _F, _F, _F, _F, _I, _VP, _VP]),
+    "spr_device_count": (_I, [ctypes.POINTER(_I)]),
+    "spr_device_name": (_I, [_I, ctypes.c_char_p, _I]),
+    "spr_loss_grad": (_I, [_VP, _I, _VP, _VP, _VP, _I, _F, _VP]),
+    "spr_refine": (_I, [_VP, _I, _F, _I, _F, _VP]),
 }
+
+
+class SprCrossDesc(ctypes.Structure):
+    """struct spr_cross_desc of include/spring_hip.h."""
+    _fields_ = [(k, ctypes.c_int) for k in ("p0", "p1", "p2", "p3", "crossft", "crosssym", "crosssizex",
+                                             "crosssizey", "squaresizex", "squaresizey", "offsetx", "offsety")] + \
+               [("rotangle", ctypes.c_float), ("crossmap_offset", ctypes.c_int), ("phase", ctypes.c_float * 8)]
+
+
+class SprOp(ctypes.Structure):
+    """struct spr_op of include/spring_hip.h."""
+    _fields_ = [("kind", ctypes.c_int), ("iterations", ctypes.c_int), ("beta", ctypes.c_float),
+                ("sigma", ctypes.c_float), ("threshold", ctypes.c_float), ("pixels", ctypes.c_int)]
+
+
+OP_KIND = {"ER": 0, "HIO": 1, "RAAR": 2, "SW": 3, "SWA": 4, "GRAD": 5}
+
+
+def make_ops(sequence):
+    """[(name, [params])] in Solver.set_sequence form -> ctypes array of SprOp.
+    ER [n]; HIO/RAAR [n, beta]; SW [sigma, threshold]; SWA [sigma, pixels]."""
+    ops = (SprOp * max(1, len(sequence)))()
+    for i, (name, p) in enumerate(sequence):
+        o = ops[i]
+        o.kind = OP_KIND[name]
+        if name == "ER":
+            o.iterations = int(p[0])
+        elif name in ("HIO", "RAAR"):
+            o.iterations, o.beta = int(p[0]), float(p[1])
+        elif name == "SW":
+            o.sigma, o.threshold = float(p[0]), float(p[1])
+        elif name == "GRAD":
+            o.iterations, o.beta, o.threshold = int(p[0]), float(p[1]), 1e-3
+            o.pixels = int(p[2]) if len(p) > 2 else 0
+        else:
+            o.sigma, o.pixels = float(p[0]), int(p[1])
+    return ops
 
 KIND_COL_PROJ, KIND_ROW_UPDATE, KIND_ROW_OTHER, KIND_COL_OTHER = 0, 1, 2, 3
 
@@ -149,6 +200,67 @@ class Backend:
 
     def sync(self):
         self._check(self.lib.spr_sync(self.h))
+
+    def sw(self, sigma, threshold, reality=0.0):
+        self._check(self.lib.spr_sw(self.h, float(sigma), float(threshold), float(reality)))
+
+    def swa(self, sigma, pixels, reality=0.0):
+        self._check(self.lib.spr_swa(self.h, float(sigma), int(pixels), float(reality)))
+
+    def evaluate(self, reality=0.0, symmetry=True):
+        """Per individual: dict of error_fourier, error_real, complexsum, modulesum[, symmetry]."""
+        P = self.P
+        out = {k: np.zeros(P, np.float32) for k in ("error_fourier", "error_real", "complexsum", "modulesum")}
+        sym = np.zeros(P, np.float32) if symmetry else None
+        self._check(self.lib.spr_evaluate(self.h, float(reality), _ptr(out["error_fourier"]), _ptr(out["error_real"]),
+                                          _ptr(out["complexsum"]), _ptr(out["modulesum"]), _ptr(sym)))
+        if symmetry:
+            out["symmetry"] = sym
+        return out
+
+    def symmetry(self, symmin=0.0, symmax=1.0):
+        s = np.zeros(self.P, np.float32)
+        self._check(self.lib.spr_symmetry(self.h, float(symmin), float(symmax), _ptr(s)))
+        return s
+
+    def run(self, sequence, nrep=1, reality=0.0):
+        ops = make_ops(sequence)
+        self._check(self.lib.spr_run(self.h, ctypes.cast(ops, ctypes.c_void_p), len(sequence), int(nrep),
+                                     float(reality)))
+
+    def align(self, ref_rho, phase_direct, phase_flip, minr=0.0, maxr=2.0):
+        """Align every resident individual to ref_rho; returns (P, 3) shifts (sx, sy, flip)."""
+        ref = np.ascontiguousarray(ref_rho, dtype=np.complex64)
+        pd = np.ascontiguousarray(phase_direct, dtype=np.float32)
+        pf = np.ascontiguousarray(phase_flip, dtype=np.float32)
+        sh = np.zeros((self.P, 3), np.int32)
+        self._check(self.lib.spr_align(self.h, _ptr(ref), float(minr), float(maxr), _ptr(pd), _ptr(pf), _ptr(sh)))
+        return sh
+
+    def average(self, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        rho = np.zeros((self.n, self.n), np.complex64)
+        s = np.zeros((self.n, self.n), np.int32)
+        self._check(self.lib.spr_average(self.h, _ptr(idx), len(idx), _ptr(rho), _ptr(s)))
+        return rho, s
+
+    def average_module(self, idx, ft=True):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        out = np.zeros((self.n, self.n), np.float32)
+        self._check(self.lib.spr_average_module(self.h, _ptr(idx), len(idx), int(bool(ft)), _ptr(out)))
+        return out
+
+    def loss_grad(self, P, rho_ptr, grad_ptr, kind=0, eps=1e-3, stream=None, want_loss=True):
+        """Loss and gradient of P fields at device addresses rho_ptr -> grad_ptr."""
+        loss = np.zeros(P, np.float64) if want_loss else None
+        self._check(self.lib.spr_loss_grad(self.h, int(P), ctypes.c_void_p(rho_ptr), ctypes.c_void_p(grad_ptr),
+                                           _ptr(loss), int(kind), float(eps), ctypes.c_void_p(stream or 0)))
+        return loss
+
+    def refine(self, steps, lr=1e-2, kind=0, eps=1e-3):
+        loss = np.zeros(self.P, np.float64)
+        self._check(self.lib.spr_refine(self.h, int(steps), float(lr), int(kind), float(eps), _ptr(loss)))
+        return loss
 
     def timing(self, enable: bool):
         self._check(self.lib.spr_timing(self.h, int(bool(enable))))

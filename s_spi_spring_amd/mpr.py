@@ -1,0 +1,194 @@
+"""Memetic phase retrieval driver (MPR) on the MI355X backend.
+
+The generation loop of the reference's spring/mpr.py:205-383 over the Solver
+of spring_core.phase_retrieval: initialise and evaluate the population, then per
+generation shrink-wrap, sort, align and average (mpr.py:426-441), crossover,
+run the algorithm sequence on parents and children, and replace a parent by its
+child when the child's error is lower (mpr.py:338-347).
+
+Besides the reference's behaviour, run() records per generation the wall-clock
+time and the best Fourier error in `self.history`, and stops early when an
+optional `target_error` is reached (time-to-R_F measurements).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from .alg_manager import ER, HIO, RAAR, AlgManager
+from .stats import Stats
+
+
+class StopRec(Exception):
+    pass
+
+
+class MPR:
+    """Memetic phase retrieval of one pattern.
+
+    :param pattern: a Pattern (or any object with get(), pid and metadata) holding the
+        centred intensity with -1 for masked and -2 for saturated pixels.
+    :param settings: a Settings object.
+    """
+
+    def __init__(self, pattern, settings, tag: str = ""):
+        from . import spring_core  # the HIP extension; fails loudly when it is not built
+        self._core = spring_core
+        self.verbose = False
+        self.fullsettings = settings
+        self.settings = settings.get()
+        self.settings_adv = settings.get_advanced()
+        self.pattern = pattern.get()
+        self.pid = getattr(pattern, "pid", [0])
+        self.tag = tag
+        self.metadata = getattr(pattern, "metadata", {})
+        self.gpulist = []
+        self.reconstructions = None
+        self.refine = False
+        self.result = None
+        self.stopevent = None
+        self.logqueue = None
+        self.history = []
+
+    def print(self, msg, end="\n", flush=True):
+        if self.logqueue is None:
+            print(msg, end=end, flush=flush)
+        else:
+            self.logqueue.put(msg + end)
+
+    # --- set-up (mpr.py:82-178) ---------------------------------------------------
+    def SelectGPUs(self):
+        g = self.settings["global"]["gpus"]
+        if isinstance(g, int):
+            ids = list(self._core.GPUtils.get_ids())
+            self.gpulist = ids if (g <= 0 or g > len(ids)) else ids[:g]
+        else:
+            self.gpulist = list(g)
+
+    def InitSolver(self):
+        Solver = self._core.phase_retrieval.Solver
+        self.solver = Solver(np.copy(self.pattern), reality=self.settings["IA"]["reality"], realpen=0,
+                             use_bounds=bool(self.settings["global"]["bounds"]), gpus=list(self.gpulist),
+                             nthreads=self.settings["global"]["threads"] * max(1, len(self.gpulist)),
+                             seed=self.settings_adv["seed"])
+        self.solver.set_shift_param(shiftmin=self.settings_adv["shift_min"], shiftmax=self.settings_adv["shift_max"])
+
+    def InitIA(self):
+        ia, adv = self.settings["IA"], self.settings_adv
+        cls = HIO if ia["alg"] == "HIO" else RAAR
+        main = [cls(ia["it"], 0, ia["beta"], decay_exp=adv["decay_exp"]),
+                ER(ia["it_ER"], ia["it_ER"] + ia["it"], decay_exp=adv["decay_exp"])]
+        stab = [ER(ia["it_stab"])] if ia["it_stab"] > 0 else []
+        gens = self.settings["global"]["generations"]
+        self.algman = AlgManager(main=main, stabilization=stab, evaluation=[ER(ia["it_eval"])],
+                                 sw_sigma=ia["sigma"], sw_threshold=ia["threshold"], repetitions=ia["repetitions"],
+                                 scalingstart=int(np.round(adv["decay_start"] * gens)),
+                                 scalingend=int(np.round(adv["decay_end"] * gens)))
+
+    def InitPop(self):
+        init, adv = self.settings["init"], self.settings_adv
+        s = init["supportsize"]
+        self.reconstructions = self.solver.InitializePop(
+            popsize=self.settings["global"]["popsize"], suppsize=s,
+            itemsize=[s * init["itemsize_min"], s * init["itemsize_max"]],
+            nitems=[init["itemnum_min"], init["itemnum_max"]], exponent=[init["gamma"], init["gamma"]],
+            scaling=[adv["scaling_min"], adv["scaling_max"]], phaserange=init["phaserange"], suppcut=0)
+
+    # --- helpers (mpr.py:426-445) -----------------------------------------------
+    def sort(self):
+        self.sortedargs = np.argsort([r.get_error() for r in self.reconstructions])
+
+    def get_best(self):
+        self.bestrec = self.reconstructions[self.sortedargs[0]]
+
+    def get_average(self):
+        self.solver.set_shift_param(shiftmin=0, shiftmax=2)
+        self.reconstructions = self.solver.Reshift(self.reconstructions, self.bestrec)
+        self.avgrec = self.solver.GetAverage(self.reconstructions, fraction=self.settings_adv["avg_frac"])
+        self.solver.set_shift_param(shiftmin=self.settings_adv["shift_min"], shiftmax=self.settings_adv["shift_max"])
+        self.solver.Evaluate(self.avgrec)
+
+    def checkstop(self):
+        return self.stopevent is not None and self.stopevent.is_set()
+
+    def save_result(self):
+        self.stats.add(self.reconstructions, self.bestrec, self.avgrec, self.replcount, self.generation,
+                       tstamp=time.time())
+        self.result = {"best_density": self.bestrec.get_density(), "best_support": self.bestrec.get_support(),
+                       "avg_density": self.avgrec.get_density(), "avg_support": self.avgrec.get_support(),
+                       "stats": self.stats.get(), "generation": self.generation}
+
+    def _record(self, t0):
+        best = min(self.reconstructions, key=lambda r: r.get_error())
+        self.history.append({"generation": self.generation, "time": time.perf_counter() - t0,
+                             "best_error": float(best.get_error()),
+                             "best_fourier_error": float(best.get_fourier_error())})
+
+    # --- main loop (mpr.py:205-383) --------------------------------------------
+    def run(self, save_every: int = -1, target_error: float = None, log: bool = True):
+        """Run the memetic loop; returns the result dictionary (best and average
+        reconstructions and the statistics). With target_error, stop after the
+        first generation whose best Fourier error is below it."""
+        t0 = time.perf_counter()
+        self.stats = Stats()
+        self.SelectGPUs()
+        self.InitSolver()
+        self.InitIA()
+        self.InitPop()
+        s = self.solver
+        s.Evaluate(self.reconstructions)
+        self.sort()
+        best = self.reconstructions[self.sortedargs[0]]
+        area = int(np.sum(best.get_support()))
+        s.set_sequence(self.algman.get_sequence(0, area=[area, area], update_support=not self.refine,
+                                                update_support_first=self.settings_adv["sw_first"]))
+        self.reconstructions = s.Sequence(self.reconstructions)
+        self.generation = 0
+        self.replcount = 0
+        self.sort()
+        self.get_best()
+        self.get_average()
+        self.save_result()
+        self._record(t0)
+        gens = self.settings["global"]["generations"]
+        while self.generation < gens:
+            if target_error is not None and self.history[-1]["best_fourier_error"] < target_error:
+                break
+            self.generation += 1
+            if not self.refine:
+                self.reconstructions = s.SW(self.reconstructions, *self.algman.get_support_params(self.generation))
+            self.sort()
+            self.get_best()
+            self.get_average()
+            if not self.refine:
+                best_px = int(np.sum(self.bestrec.get_support()))
+                eval_px = int(np.sum(self.avgrec.get_support()))
+                if eval_px > best_px:
+                    eval_px = best_px + int(self.settings_adv["avg_supp"] * (eval_px - best_px))
+                suppix = [best_px, eval_px]
+            else:
+                suppix = [0, 0]
+            ga = self.settings["GA"]
+            children = s.Crossover(self.reconstructions, self.bestrec, self.avgrec, ga["crossprob"],
+                                   ga["crossweight"], self.settings_adv["crossexp"], self.settings_adv["crossft"],
+                                   self.settings_adv["crosssym"], ga["crossaverage"], update_support=not self.refine)
+            s.set_sequence(self.algman.get_sequence(self.generation, area=suppix, update_support=not self.refine))
+            self.reconstructions = s.Sequence(self.reconstructions)
+            children = s.Sequence(children)
+            repl = np.zeros(len(self.reconstructions))
+            for i, child in enumerate(children):
+                j = i % len(self.reconstructions)
+                if child.get_error() < self.reconstructions[j].get_error():
+                    repl[j] = 1
+                    self.reconstructions[j] = child
+            self.replcount = int(np.sum(repl))
+            self.save_result()
+            self._record(t0)
+            if log:
+                h = self.history[-1]
+                self.print(f"gen {self.generation:4d}  best E_F {h['best_fourier_error']:.5f}  "
+                           f"replaced {self.replcount}  t {h['time']:.2f} s")
+            if self.checkstop():
+                break
+        return self.result

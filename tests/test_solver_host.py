@@ -1,0 +1,73 @@
+"""Host-side parts of spring_core.phase_retrieval (no GPU needed): the binding's
+surface (names, defaults, single-or-list convention), the pattern preparation
+(Solver::set_boundaries + prepare, src/solver.cpp:173-289) against the oracle,
+and the error behaviour."""
+import inspect
+
+import numpy as np
+import pytest
+
+from s_spi_spring_amd import synth
+
+
+@pytest.fixture(scope="module")
+def core():
+    from s_spi_spring_amd import spring_core
+    return spring_core
+
+
+def test_submodules_importable(core):
+    import importlib
+    pr = importlib.import_module("s_spi_spring_amd.spring_core.phase_retrieval")
+    assert pr.Solver is core.phase_retrieval.Solver
+    assert hasattr(core.GPUtils, "get_ids") and hasattr(core.GPUtils, "get_info")
+
+
+def test_reconstruction_roundtrip(core):
+    R = core.phase_retrieval.Reconstruction
+    d = (np.arange(16) + 1j * np.arange(16)).reshape(4, 4).astype(np.complex64)
+    s = (np.arange(16) % 2).reshape(4, 4).astype(np.int32)
+    r = R(d, s)
+    assert np.array_equal(r.get_density(), d) and np.array_equal(r.get_support(), s)
+    assert r.get_error() == -1.0 and r.get_real_error() == -1.0
+    r.set_density(d * 2)
+    r.set_support(1 - s)
+    assert np.array_equal(r.get_density(), d * 2) and np.array_equal(r.get_support(), 1 - s)
+    with pytest.raises(Exception):
+        r.set_support(np.zeros((3, 3), np.int32))
+
+
+@pytest.mark.parametrize("use_bounds", [False, True])
+def test_pattern_preparation_matches_oracle(core, orc, use_bounds):
+    I, _ = synth.ellipsoids_pattern(seed=2, n=256)
+    I[10, 10] = -2.0   # a saturated pixel
+    S = core.phase_retrieval.Solver(I, use_bounds=use_bounds, gpus=[0], nthreads=2)
+    got = S.get_pattern()
+    want = orc.prepare_pattern(I, use_bounds=use_bounds)
+    assert got.dtype == np.float32
+    assert np.array_equal(got, want)
+
+
+def test_solver_requires_a_device(core):
+    with pytest.raises(Exception, match="no CPU backend"):
+        core.phase_retrieval.Solver(np.ones((64, 64), np.float32))
+    with pytest.raises(Exception, match="unsupported pattern size"):
+        core.phase_retrieval.Solver(np.ones((100, 100), np.float32), gpus=[0])
+
+
+def test_binding_signature_defaults(core):
+    doc = core.phase_retrieval.Solver.__init__.__doc__
+    for token in ("pattern", "center", "reality", "realpen", "nthreads", "use_bounds", "gpus", "seed"):
+        assert token in doc
+    assert doc.split("seed")[1].split(")")[0].strip().endswith("= 17")
+    cdoc = core.phase_retrieval.Solver.Crossover.__doc__
+    for token in ("crossbalance", "crossdiff", "crossexp", "crossft", "crosssym", "usebest", "update_support"):
+        assert token in cdoc
+
+
+def test_set_sequence_skips_unknown(core, capfd):
+    S = core.phase_retrieval.Solver(np.ones((64, 64), np.float32), gpus=[0], nthreads=1)
+    S.set_sequence([("HIO", [10.0, 0.9]), ("FOO", [1.0]), ("ER", [1.0, 2.0])])
+    out = capfd.readouterr().out
+    assert "FOO [ 1 ]  NOT RECOGNIZED. Skipping." in out
+    assert "ER [ 1 2 ]  NOT RECOGNIZED. Skipping." in out
