@@ -15,6 +15,8 @@ oracle timed on this host as the CPU baseline.
 from __future__ import annotations
 
 import argparse
+import csv
+import hashlib
 import json
 import os
 import platform
@@ -28,6 +30,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "individual-iters/s at 512² pop64; time-to-Rf<0.05, batched+grad vs plain GPU"
+ROW_MODE = {"HIO": 2, "RAAR": 3, "ER": 4}       # RowMode of spr_kernels.hpp
+DEVICE_SOURCES = ("spr_fft.hpp", "spr_kernels.hpp", "spr_aux.hpp", "spr_grad.hpp", "spr_backend.hip")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_iteration_512_pop64_raar.json")
+COUNTER_CMD = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 90 rocprofv3 --pmc {ctr} --kernel-trace "
+               "--output-format csv -d <dir> -o run -- python3 -u /root/repo/bench.py --steps 2 --warmup 1 "
+               "--no-cpu --calibrate {mib}")
 
 
 def algorithmic_bytes(n, P, alg):
@@ -88,6 +96,119 @@ def copy_calibration(mib, log):
     log(f"copy calibration: 3 device-to-device copies of {mib} MiB")
 
 
+def device_source_hash():
+    """Hash of the device sources: a traffic summary applies to the kernels it was
+    measured on only."""
+    h = hashlib.sha256()
+    for f in DEVICE_SOURCES:
+        with open(os.path.join(ROOT, "s_spi_spring_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def rocprof_kernel_name(kernel, n, alg):
+    if kernel == "row":
+        return f"void spr::k_row<{n}, {ROW_MODE[alg]}>(spr::RowArgs)"
+    return f"void spr::k_col<{n}, 0>(spr::ColArgs)"
+
+
+def _counter_values(run_dir, counter):
+    vals = {}
+    with open(os.path.join(run_dir, "run_counter_collection.csv")) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] == counter:
+                vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return vals
+
+
+def summarize_traffic(fetch_dir, write_dir, mib, out_path, log):
+    """HBM bytes per dispatch of every spr:: kernel from a FETCH_SIZE and a
+    WRITE_SIZE counter run of `bench.py --calibrate MIB`. Each counter is scaled
+    so that it reads MIB MiB for the three MIB MiB device-to-device copies (the
+    largest copy dispatches of the run)."""
+    fetch = _counter_values(fetch_dir, "FETCH_SIZE")
+    write = _counter_values(write_dir, "WRITE_SIZE")
+
+    def factor(vals):
+        copies = sorted(v for k, lst in vals.items() if k.startswith("__amd_rocclr_copyBuffer") for v in lst)[-3:]
+        if len(copies) < 3:
+            raise SystemExit("the counter runs hold no calibration copies (run them with --calibrate MIB)")
+        return mib * 1024.0 / (sum(copies) / len(copies))
+
+    ff, fw = factor(fetch), factor(write)
+    kernels = {}
+    for k in sorted(fetch):
+        if not k.startswith("void spr::") or k not in write:
+            continue
+        f_kib = sum(fetch[k]) / len(fetch[k])
+        w_kib = sum(write[k]) / len(write[k])
+        kernels[k] = {"dispatches": len(fetch[k]), "fetch_kib": round(f_kib, 1), "write_kib": round(w_kib, 1),
+                      "hbm_bytes": int(round((f_kib * ff + w_kib * fw) * 1024))}
+    out = {"what": "HBM bytes per dispatch (FETCH_SIZE x fetch_factor + WRITE_SIZE x write_factor, KiB x 1024)",
+           "commands": [COUNTER_CMD.format(ctr="FETCH_SIZE", mib=mib), COUNTER_CMD.format(ctr="WRITE_SIZE", mib=mib)],
+           "device_source_hash": device_source_hash(),
+           "calibration": {"copy_mib": mib, "fetch_factor": round(ff, 4), "write_factor": round(fw, 4)},
+           "kernels": kernels}
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+        f.write("\n")
+    for k, v in kernels.items():
+        log(f"{k}: {v['hbm_bytes'] / 1e6:.1f} MB per dispatch over {v['dispatches']} dispatches")
+    log(f"wrote {out_path}")
+
+
+def measured_traffic(kernel_name, log):
+    """HBM bytes per launch of `kernel_name` from the committed counter summary,
+    or None when there is none or it was measured on other device sources."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except OSError:
+        log("no HBM traffic summary: roofline.traffic is null")
+        return None
+    if t.get("device_source_hash") != device_source_hash():
+        log("HBM traffic summary was measured on other device sources: roofline.traffic is null")
+        return None
+    k = t["kernels"].get(kernel_name)
+    return None if k is None else k["hbm_bytes"]
+
+
+def time_to_rf_study(seeds, generations, target, log):
+    """Time to best E_F < target of the plain and the gradient schedules over
+    `seeds` seeded 512^2 ellipsoid patterns (population 64); one JSON line."""
+    from s_spi_spring_amd import workloads
+    log("warm-up run (not recorded)")
+    workloads.time_to_rf(10_000, "plain", target=target, generations=0)
+    runs = {"plain": [], "grad": []}
+    for seed in range(seeds):
+        for form in ("plain", "grad"):
+            r = workloads.time_to_rf(seed, form, target=target, generations=generations)
+            runs[form].append(r)
+            log(f"seed {seed:2d} {form:5s}: t={r['time_to_target']} s gen={r['generation']} "
+                f"E_F={r['fourier_error']:.4f} real-space={r['real_space_error']:.4f} wall={r['wall']:.1f} s")
+
+    def summary(rs):
+        t = [r["time_to_target"] for r in rs if r["time_to_target"] is not None]
+        return {"reached": len(t), "runs": len(rs),
+                "median_time_s": float(np.median(t)) if t else None,
+                "mean_time_s": float(np.mean(t)) if t else None,
+                "median_generations": float(np.median([r["generation"] for r in rs])),
+                "median_fourier_error": float(np.median([r["fourier_error"] for r in rs])),
+                "median_real_space_error": float(np.median([r["real_space_error"] for r in rs])),
+                "real_space_error_q25_q75": [float(np.percentile([r["real_space_error"] for r in rs], q))
+                                             for q in (25, 75)]}
+    sp, sg = summary(runs["plain"]), summary(runs["grad"])
+    speedup = (sp["median_time_s"] / sg["median_time_s"]) if sp["median_time_s"] and sg["median_time_s"] else None
+    out = {"metric": f"time-to-Rf<{target}", "unit": "s", "higher_is_better": False, "seeds": seeds,
+           "generations_max": generations, "config": {"workload": "ellipsoids 512^2 pattern, 1e9 photons, "
+                                                      "population 64, RAAR400+HIO100+ER50 per generation in "
+                                                      "25-iteration blocks with shrink-wrap"},
+           "plain": sp, "grad": sg, "speedup_grad_vs_plain": speedup,
+           "per_seed": {f: [{k: r[k] for k in ("seed", "time_to_target", "generation", "fourier_error",
+                                               "real_space_error")} for r in rs] for f, rs in runs.items()}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,15 +223,31 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", type=int, default=0, metavar="MIB",
                     help="also run device-to-device copies of MIB MiB (counter calibration)")
+    ap.add_argument("--time-to-rf", type=int, default=0, metavar="SEEDS",
+                    help="instead of the throughput step: time to best E_F < --target of the plain and "
+                         "gradient schedules over SEEDS patterns (one JSON line)")
+    ap.add_argument("--target", type=float, default=0.05)
+    ap.add_argument("--generations", type=int, default=10)
+    ap.add_argument("--summarize-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write the HBM traffic summary from two rocprofv3 --pmc runs of "
+                         "`bench.py --calibrate MIB` (FETCH_SIZE, WRITE_SIZE) and exit; no GPU needed")
     args = ap.parse_args()
 
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
+    if args.summarize_traffic:
+        summarize_traffic(*args.summarize_traffic, args.calibrate or 256, TRAFFIC_FILE, log)
+        return
+
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the population "
               "shards by individuals across GPUs; only the one-GPU path is built).", file=sys.stderr)
         sys.exit(2)
+
+    if args.time_to_rf:
+        time_to_rf_study(args.time_to_rf, args.generations, args.target, log)
+        return
 
     from s_spi_spring_amd import hipcore, synth
 
@@ -151,10 +288,10 @@ def main():
     row_gbs = row_b / row_avg / 1e9
     # dominant kernel = the one with the larger share of the timed region
     if row_ms >= col_ms:
-        dom = {"kernel": f"k_row<{n},{args.alg}>", "achieved": row_gbs, "bytes_per_launch": row_b,
+        dom = {"kernel": rocprof_kernel_name("row", n, args.alg), "achieved": row_gbs, "bytes_per_launch": row_b,
                "avg_launch_us": row_avg * 1e6}
     else:
-        dom = {"kernel": f"k_col<{n},PROJ>", "achieved": col_gbs, "bytes_per_launch": col_b,
+        dom = {"kernel": rocprof_kernel_name("col", n, args.alg), "achieved": col_gbs, "bytes_per_launch": col_b,
                "avg_launch_us": col_avg * 1e6}
     log(f"column pass: {col_cnt} launches, avg {col_avg * 1e6:.1f} us, {col_gbs:.0f} GB/s algorithmic")
     log(f"row pass:    {row_cnt} launches, avg {row_avg * 1e6:.1f} us, {row_gbs:.0f} GB/s algorithmic")
@@ -163,7 +300,8 @@ def main():
         f"({value * iter_bytes / 1e9 / HBM_PEAK_GBS:.3f} of HBM peak)")
 
     roofline = {"bound": "hbm", "achieved": round(dom["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dom["achieved"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(dom["achieved"] / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(dom["kernel"], log),
                 "kernel": dom["kernel"], "algorithmic_bytes_per_launch": dom["bytes_per_launch"],
                 "avg_launch_us": round(dom["avg_launch_us"], 2),
                 "iteration_frac": round(value * iter_bytes / 1e9 / HBM_PEAK_GBS, 4)}

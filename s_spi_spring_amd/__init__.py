@@ -19,6 +19,15 @@ from __future__ import annotations
 
 import sys
 
+try:
+    # libspring_hip.so binds to the HIP runtime loaded under the soname
+    # `libamdhip64.so`; importing torch first makes that torch's runtime, so that
+    # torch tensors and this library share one HIP/HSA runtime and one device
+    # context in the process (two runtimes cannot both open the GPU).
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 from .alg_manager import ER, HIO, RAAR, SW, AlgManager  # noqa: F401
 from .pattern import Pattern  # noqa: F401
 from .settings import Settings  # noqa: F401

@@ -1,6 +1,8 @@
 """In-tree build of the native parts of s_spi_spring_amd.
 
-  lib/libspring_hip.so   HIP kernels + C ABI (include/spring_hip.h), gfx950 only
+  lib/libspring_hip.so   HIP kernels + C ABI (include/spring_hip.h), gfx950 only;
+                         needs the HIP runtime by the soname `libamdhip64.so`
+                         (RUNPATH /opt/rocm/lib), see _runtime_stub
 
 Run ``python -m s_spi_spring_amd.build`` or call ``build_all()``. The build runs
 without a GPU (hipcc cross-compiles for gfx950).
@@ -31,15 +33,51 @@ def _run(cmd):
     return r
 
 
-def build_hip(verbose: bool = False) -> str:
-    os.makedirs(LIBDIR, exist_ok=True)
-    out = os.path.join(LIBDIR, "libspring_hip.so")
-    tmp = out + ".tmp"
-    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-I", INCLUDE, "-I", CSRC,
-           "-o", tmp, os.path.join(CSRC, "spr_backend.hip")]
+ROCM_LIB = os.environ.get("ROCM_LIB", "/opt/rocm/lib")
+
+
+def _runtime_stub(obj: str, builddir: str, verbose: bool) -> None:
+    """Link-time stand-in for the HIP runtime with soname `libamdhip64.so`.
+
+    PyTorch-ROCm wheels ship their own HIP runtime under that unversioned soname,
+    while /opt/rocm's carries `libamdhip64.so.7`. Two HIP (and HSA) runtimes in one
+    process do not share the device: the second one to initialise finds no GPU.
+    Linking against a stub with the unversioned soname makes the dynamic loader
+    bind libspring_hip.so to a runtime the process has already loaded under that
+    name (torch's, when torch was imported first), and otherwise load
+    /opt/rocm/lib/libamdhip64.so through the RUNPATH. The stub defines the HIP
+    entry points the object file references and is deleted after the link."""
+    r = _run(["nm", "-u", "--format=posix", obj])
+    syms = sorted({ln.split()[0] for ln in r.stdout.splitlines()
+                   if ln.split() and ln.split()[0].startswith(("hip", "__hip"))})
+    src = os.path.join(builddir, "hip_stub.c")
+    with open(src, "w") as f:
+        f.write("".join(f"void {s}(void) {{}}\n" for s in syms))
+    cmd = ["gcc", "-shared", "-fPIC", "-Wl,-soname,libamdhip64.so", "-o",
+           os.path.join(builddir, "libamdhip64.so"), src]
     if verbose:
         print(" ".join(cmd))
     _run(cmd)
+
+
+def build_hip(verbose: bool = False) -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    builddir = os.path.join(LIBDIR, "_build")
+    os.makedirs(builddir, exist_ok=True)
+    out = os.path.join(LIBDIR, "libspring_hip.so")
+    obj = os.path.join(builddir, "spr_backend.o")
+    cmd = [HIPCC, *HIP_FLAGS, "-c", "-I", INCLUDE, "-I", CSRC, "-o", obj, os.path.join(CSRC, "spr_backend.hip")]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    _runtime_stub(obj, builddir, verbose)
+    tmp = out + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-shared", "-fPIC", "-o", tmp, obj, "-L", builddir, "-lamdhip64",
+           "-Wl,-z,defs", "-Wl,--enable-new-dtags", "-Wl,-rpath," + ROCM_LIB]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.remove(os.path.join(builddir, "libamdhip64.so"))
     os.replace(tmp, out)
     return out
 

@@ -149,13 +149,13 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row_aux(AuxRowArgs a) {
     using G = Geo<N>;
     constexpr int T = G::T, NT = G::ROW_THREADS;
     constexpr int WG_PER_FIELD = N / G::RW;
-    __shared__ float2 tw[N];
+    __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[G::RW * G::RS];
     __shared__ double red[4 * (NT / 64)];
     const int tid = threadIdx.x;
     const int t = tid % T;
     const int rl = tid / T;
-    for (int i = tid; i < N; i += NT) tw[i] = a.tw[i];
+    tw_fill<N>(tw, a.tw, tid, NT);
     const int item = blockIdx.x / WG_PER_FIELD;
     const int y = (blockIdx.x % WG_PER_FIELD) * G::RW + rl;
     const size_t base = (size_t)item * N * N + (size_t)y * N;
@@ -292,13 +292,13 @@ template <int N, int MODE>
 __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_aux(AuxColArgs a) {
     using G = Geo<N>;
     constexpr int T = G::T, C = G::C, NT = G::COL_THREADS;
-    __shared__ float2 tw[N];
+    __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[N * G::CS];
     __shared__ double red[4 * (NT / 64)];
     const int tid = threadIdx.x;
     const int c = tid % C;
     const int t = tid / C;
-    for (int i = tid; i < N; i += NT) tw[i] = a.tw[i];
+    tw_fill<N>(tw, a.tw, tid, NT);
     constexpr int TILES = N / C;
     const int item = blockIdx.x / TILES;
     const int x0 = (blockIdx.x % TILES) * C;
@@ -513,10 +513,10 @@ template <int N, int SIGN>
 __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_fft_rows(FieldBatch b) {
     using G = Geo<N>;
     constexpr int T = G::T, NT = G::ROW_THREADS, WPF = N / G::RW;
-    __shared__ float2 tw[N];
+    __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[G::RW * G::RS];
     const int tid = threadIdx.x, t = tid % T, rl = tid / T;
-    for (int i = tid; i < N; i += NT) tw[i] = b.tw[i];
+    tw_fill<N>(tw, b.tw, tid, NT);
     const int item = blockIdx.x / WPF;
     const int y = (blockIdx.x % WPF) * G::RW + rl;
     const size_t fo = field_of(b, item) * N * N + (size_t)y * N;
@@ -537,10 +537,10 @@ template <int N, int SIGN>
 __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_fft_cols(FieldBatch b) {
     using G = Geo<N>;
     constexpr int T = G::T, C = G::C, NT = G::COL_THREADS, TILES = N / C;
-    __shared__ float2 tw[N];
+    __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[N * G::CS];
     const int tid = threadIdx.x, c = tid % C, t = tid / C;
-    for (int i = tid; i < N; i += NT) tw[i] = b.tw[i];
+    tw_fill<N>(tw, b.tw, tid, NT);
     const int item = blockIdx.x / TILES;
     const int x0 = (blockIdx.x % TILES) * C;
     float2* __restrict__ f = b.base + field_of(b, item) * N * N + x0;
@@ -581,10 +581,10 @@ template <int N>
 __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_corr_spec_cols(CorrSpecArgs a) {
     using G = Geo<N>;
     constexpr int T = G::T, C = G::C, NT = G::COL_THREADS, TILES = N / C;
-    __shared__ float2 tw[N];
+    __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[N * G::CS];
     const int tid = threadIdx.x, c = tid % C, t = tid / C;
-    for (int i = tid; i < N; i += NT) tw[i] = a.tw[i];
+    tw_fill<N>(tw, a.tw, tid, NT);
     const int item = blockIdx.x / TILES;
     const int x0 = (blockIdx.x % TILES) * C;
     const float2* __restrict__ d = a.spec + (size_t)a.d_index[item] * N * N + x0;

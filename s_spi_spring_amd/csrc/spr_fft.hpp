@@ -11,8 +11,12 @@
 // can therefore be chained in registers, with a per-element operation between
 // them, without touching LDS.
 //
-// Twiddles come from a per-workgroup LDS copy of root[m] = exp(-2*pi*i*m/N),
-// computed in double on the host and rounded to float.
+// Twiddles come from a per-workgroup LDS table laid out per stage
+// (TwTable<N>): for stage s with span Ns and radix R, row k in [0, Ns) holds
+// w^(r*k), r = 1..R-1, densely at the odd stride R-1 so that the 64 lanes of a
+// wave, which read consecutive k, spread over the LDS banks. The table is built from
+// root[m] = exp(-2*pi*i*m/N), computed in double on the host and rounded to
+// float, so every twiddle equals the float rounding of the exact root.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -87,24 +91,60 @@ template <int N> constexpr int span_before(int s) {
     return ns;
 }
 
+// Per-stage twiddle tables in LDS. Stage s >= 1 (radix R, span NS) needs
+// w^(r*k) = root[r*k*N/(NS*R)] for k < NS, 1 <= r < R; they are stored densely
+// at tws[off(s) + k*(R-1) + r-1]. R is a power of two, so the row stride R-1 is
+// odd and the 8-byte reads of 16 consecutive k fall in distinct LDS bank pairs;
+// one base address serves all r.
+template <int N> struct TwTable {
+    static constexpr int off(int s) {
+        int o = 0;
+        for (int i = 1; i < s; ++i) o += span_before<N>(i) * (RadixPlan<N>::r[i] - 1);
+        return o;
+    }
+    static constexpr int SIZE = off(RadixPlan<N>::ns);
+};
+
+template <int N, int S>
+__device__ __forceinline__ void tw_fill_stage(float2* tws, const float2* __restrict__ root, int tid, int nt) {
+    if constexpr (S < RadixPlan<N>::ns) {
+        constexpr int R = RadixPlan<N>::r[S];
+        constexpr int NS = span_before<N>(S);
+        constexpr int STEP = N / (NS * R);
+        constexpr int OFF = TwTable<N>::off(S);
+        for (int e = tid; e < NS * (R - 1); e += nt) {
+            const int k = e / (R - 1), r = e % (R - 1) + 1;
+            tws[OFF + k * (R - 1) + r - 1] = root[r * k * STEP];
+        }
+        tw_fill_stage<N, S + 1>(tws, root, tid, nt);
+    }
+}
+
+// Fill the per-stage tables from the global root table exp(-2 pi i m/N) (the
+// caller synchronises before the first transform).
+template <int N>
+__device__ __forceinline__ void tw_fill(float2* tws, const float2* __restrict__ root, int tid, int nt) {
+    tw_fill_stage<N, 1>(tws, root, tid, nt);
+}
+
 // Twiddle and butterfly of stage STAGE on registers v (8 values = 8/R butterflies).
 template <int N, int STAGE, int SIGN>
-__device__ __forceinline__ void stage_compute(float2 (&v)[8], const float2* __restrict__ tw, int t) {
+__device__ __forceinline__ void stage_compute(float2 (&v)[8], const float2* __restrict__ tws, int t) {
     constexpr int R = RadixPlan<N>::r[STAGE];
     constexpr int NS = span_before<N>(STAGE);
     constexpr int T = N / 8;
     constexpr int NB = 8 / R;
-    constexpr int STEP = N / (NS * R);
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
         if constexpr (NS > 1) {
             const int b = t + T * u;
             const int k = b & (NS - 1);
+            const float2* w = tws + TwTable<N>::off(STAGE) + k * (R - 1);
 #pragma unroll
             for (int r = 1; r < R; ++r) {
-                float2 w = tw[r * k * STEP];
-                if (SIGN > 0) w.y = -w.y;
-                v[u * R + r] = cmul(v[u * R + r], w);
+                float2 wr = w[r - 1];
+                if (SIGN > 0) wr.y = -wr.y;
+                v[u * R + r] = cmul(v[u * R + r], wr);
             }
         }
         dftR<R, SIGN>(&v[u * R]);

@@ -30,11 +30,11 @@ template <int N>
 __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_loss(LossArgs a) {
     using G = Geo<N>;
     constexpr int T = G::T, C = G::C, NT = G::COL_THREADS, TILES = N / C;
-    __shared__ float2 tw[N];
+    __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[N * G::CS];
     __shared__ double red[4 * (NT / 64)];
     const int tid = threadIdx.x, c = tid % C, t = tid / C;
-    for (int i = tid; i < N; i += NT) tw[i] = a.tw[i];
+    tw_fill<N>(tw, a.tw, tid, NT);
     const int item = blockIdx.x / TILES;
     const int x0 = (blockIdx.x % TILES) * C;
     float2* __restrict__ f = a.f + (size_t)item * N * N + x0;

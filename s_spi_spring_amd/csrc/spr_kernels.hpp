@@ -35,7 +35,7 @@ template <int N> struct Geo {
     static constexpr int COL_THREADS = C * T;
     static constexpr int CS = C + 1;                      // padded LDS row length of a column tile
     // Two column workgroups per CU when their LDS fits (160 KB): waves per SIMD to target.
-    static constexpr int COL_LDS = (N * CS + N) * 8 + 512;
+    static constexpr int COL_LDS = (N * CS + TwTable<N>::SIZE) * 8 + 512;
 #ifndef SPR_COL_WG_PER_CU
 #define SPR_COL_WG_PER_CU 2
 #endif
@@ -89,9 +89,17 @@ __device__ __forceinline__ float2 polar_of_zero(float2 F, float a) {
     return make_float2(-a, signbit(F.y) ? s * a : -s * a);
 }
 
-// Fourier modulus projection P_M (sub_intensities).
+// Fourier modulus projection P_M (sub_intensities). The measured-pixel case
+// scales by a/|F| through the hardware reciprocal square root (within 2 ulp of
+// the reference's a*F/|F|) whenever |F|^2 is a normal float; tiny, zero and
+// bounded values take the exact formula.
 __device__ __forceinline__ float2 project_modulus(float2 F, float a) {
-    const float m = sqrtf(F.y * F.y + F.x * F.x);
+    const float m2 = F.y * F.y + F.x * F.x;
+    if (a >= 0.f && m2 > 1e-36f) {
+        const float s = a * rsqrtf(m2);
+        return make_float2(F.x * s, F.y * s);
+    }
+    const float m = sqrtf(m2);
     if (a >= 0.f) {
         if (m > 0.f) return make_float2(a * F.x / m, a * F.y / m);
         return polar_of_zero(F, a);
@@ -132,12 +140,12 @@ template <int N, int MODE>
 __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
     using G = Geo<N>;
     constexpr int T = G::T;
-    __shared__ float2 tw[N];
+    __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[G::RW * G::RS];
     const int tid = threadIdx.x;
     const int t = tid % T;
     const int rl = tid / T;
-    for (int i = tid; i < N; i += G::ROW_THREADS) tw[i] = a.tw[i];
+    tw_fill<N>(tw, a.tw, tid, G::ROW_THREADS);
     // Block-uniform bases with 32-bit per-thread offsets (SGPR base addressing).
     const size_t blk = (size_t)blockIdx.x * G::RW * N;
     float2* __restrict__ rho = a.rho + blk;
@@ -218,12 +226,12 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) __attribute__((amdgpu_waves_pe
 void k_col(ColArgs a) {
     using G = Geo<N>;
     constexpr int T = G::T, C = G::C;
-    __shared__ float2 tw[N];
+    __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[N * G::CS];
     const int tid = threadIdx.x;
     const int c = tid % C;
     const int t = tid / C;
-    for (int i = tid; i < N; i += G::COL_THREADS) tw[i] = a.tw[i];
+    tw_fill<N>(tw, a.tw, tid, G::COL_THREADS);
     constexpr int TILES = N / C;
     const int p = blockIdx.x / TILES;
     const int x0 = (blockIdx.x % TILES) * C;

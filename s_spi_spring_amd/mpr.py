@@ -44,6 +44,7 @@ class MPR:
         self.tag = tag
         self.metadata = getattr(pattern, "metadata", {})
         self.gpulist = []
+        self.schedule = None
         self.reconstructions = None
         self.refine = False
         self.result = None
@@ -126,10 +127,19 @@ class MPR:
                              "best_fourier_error": float(best.get_fourier_error())})
 
     # --- main loop (mpr.py:205-383) --------------------------------------------
-    def run(self, save_every: int = -1, target_error: float = None, log: bool = True):
+    def _sequence(self, generation, area, update_support_first=False):
+        if self.schedule is not None:
+            return self.schedule(generation, area)
+        return self.algman.get_sequence(generation, area=area, update_support=not self.refine,
+                                        update_support_first=update_support_first)
+
+    def run(self, save_every: int = -1, target_error: float = None, log: bool = True, schedule=None):
         """Run the memetic loop; returns the result dictionary (best and average
         reconstructions and the statistics). With target_error, stop after the
-        first generation whose best Fourier error is below it."""
+        first generation whose best Fourier error is below it. `schedule`
+        (optional) maps (generation, [best area, average area]) to the
+        [(name, [params])] sequence of that generation instead of AlgManager."""
+        self.schedule = schedule
         t0 = time.perf_counter()
         self.stats = Stats()
         self.SelectGPUs()
@@ -141,8 +151,7 @@ class MPR:
         self.sort()
         best = self.reconstructions[self.sortedargs[0]]
         area = int(np.sum(best.get_support()))
-        s.set_sequence(self.algman.get_sequence(0, area=[area, area], update_support=not self.refine,
-                                                update_support_first=self.settings_adv["sw_first"]))
+        s.set_sequence(self._sequence(0, [area, area], self.settings_adv["sw_first"]))
         self.reconstructions = s.Sequence(self.reconstructions)
         self.generation = 0
         self.replcount = 0
@@ -173,7 +182,7 @@ class MPR:
             children = s.Crossover(self.reconstructions, self.bestrec, self.avgrec, ga["crossprob"],
                                    ga["crossweight"], self.settings_adv["crossexp"], self.settings_adv["crossft"],
                                    self.settings_adv["crosssym"], ga["crossaverage"], update_support=not self.refine)
-            s.set_sequence(self.algman.get_sequence(self.generation, area=suppix, update_support=not self.refine))
+            s.set_sequence(self._sequence(self.generation, suppix))
             self.reconstructions = s.Sequence(self.reconstructions)
             children = s.Sequence(children)
             repl = np.zeros(len(self.reconstructions))

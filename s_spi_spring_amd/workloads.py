@@ -1,0 +1,97 @@
+"""Algorithm schedules of the benchmark workloads, as [(name, [params])] lists
+for Solver.set_sequence / MPR.run(schedule=...).
+
+plain_small   128^2 polygon: 10 blocks of 20 iterations (7 HIO(beta 0.9) blocks,
+              one HIO(10)+ER(10) block, two ER blocks), shrink-wrap SW(sigma_k, 0.2)
+              after each block with sigma_k = 3 - 1.5 k/9.
+plain_512     512^2 ellipsoids: RAAR 400 (beta 0.87) + HIO 100 (beta 0.95) + ER 50 in
+              25-iteration blocks, SW after each block (22 per generation,
+              sigma 3 -> 1.5, threshold 0.2).
+grad_512      the same blocks with a 20-step Adam refinement of the masked Poisson
+              likelihood (lr 1e-2) after each RAAR block.
+
+time_to_rf runs one memetic reconstruction of the 512^2 ellipsoid pattern with
+one of these schedules and reports the wall-clock time from population
+initialisation to the first generation whose best Fourier error is below the
+target, with the real-space error of the best individual against the object.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+
+def _sigma(k, nblocks, start=3.0, end=1.5):
+    return start - (start - end) * k / max(nblocks - 1, 1)
+
+
+def plain_small(generation=0, area=None):
+    blocks = [[("HIO", [20, 0.9])]] * 7 + [[("HIO", [10, 0.9]), ("ER", [10])], [("ER", [20])], [("ER", [20])]]
+    seq = []
+    for k, b in enumerate(blocks):
+        seq += b + [("SW", [_sigma(k, len(blocks)), 0.2])]
+    return seq
+
+
+def _blocks_512(grad):
+    blocks = []
+    for _ in range(16):
+        blocks.append([("RAAR", [25, 0.87])] + ([("GRAD", [20, 1e-2])] if grad else []))
+    blocks += [[("HIO", [25, 0.95])]] * 4 + [[("ER", [25])]] * 2
+    return blocks
+
+
+def plain_512(generation=0, area=None):
+    blocks = _blocks_512(False)
+    seq = []
+    for k, b in enumerate(blocks):
+        seq += b + [("SW", [_sigma(k, len(blocks)), 0.2])]
+    return seq
+
+
+def grad_512(generation=0, area=None):
+    blocks = _blocks_512(True)
+    seq = []
+    for k, b in enumerate(blocks):
+        seq += b + [("SW", [_sigma(k, len(blocks)), 0.2])]
+    return seq
+
+
+def iterations(seq):
+    """Individual-iterations of ER/HIO/RAAR in a sequence (per individual)."""
+    return sum(int(p[0]) for name, p in seq if name in ("ER", "HIO", "RAAR"))
+
+
+SCHEDULES = {"plain": plain_512, "grad": grad_512}
+
+
+def support_size(truth):
+    """2 * (largest distance of an object pixel from the grid centre) + 4."""
+    n = truth.shape[0]
+    y, x = np.nonzero(np.abs(truth) > 0)
+    return int(np.ceil(2 * np.sqrt(((x - n / 2) ** 2 + (y - n / 2) ** 2).max()) + 4))
+
+
+def time_to_rf(seed, form="plain", target=0.05, generations=10, popsize=64, n=512, photons=1e9):
+    """One memetic run on the ellipsoid pattern of `seed`; returns the time to
+    best E_F < target (None when not reached within `generations`), the
+    generation reached, and the final Fourier and real-space errors of the
+    best individual."""
+    from . import Pattern, Settings, metrics, synth
+    from .mpr import MPR
+    I, truth = synth.ellipsoids_pattern(seed=seed, n=n, photons=photons)
+    s = Settings()
+    s.set("global", "popsize", popsize).set("global", "generations", generations)
+    s.set("init", "supportsize", support_size(truth)).set("IA", "reality", 0.0)
+    s.set_advanced("seed", 1000 + seed)
+    m = MPR(Pattern.from_intensity(I), s)
+    t0 = time.perf_counter()
+    res = m.run(target_error=target, log=False, schedule=SCHEDULES[form])
+    wall = time.perf_counter() - t0
+    h = m.history[-1]
+    reached = h["best_fourier_error"] < target
+    return {"seed": seed, "form": form, "time_to_target": h["time"] if reached else None,
+            "wall": wall, "generation": h["generation"], "fourier_error": h["best_fourier_error"],
+            "real_space_error": metrics.real_space_error(res["best_density"], truth),
+            "iterations": iterations(SCHEDULES[form]()) * (h["generation"] + 1)}

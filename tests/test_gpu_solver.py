@@ -56,12 +56,19 @@ def test_sequence_matches_oracle(pr, orc):
     S.set_sequence(seq)
     out = S.Sequence(recs)
     assert out[0] is recs[0]
+    amp64 = amp.astype(np.float64)
     for i, r in enumerate(recs):
-        rho = orc.iterate("HIO", rho0[i], sup0[i], amp, 3, 0.9, 0.0)
-        sup = orc.sw(rho.astype(np.complex128), sup0[i], amp.astype(np.float64), 2.5, 0.1)
-        rho = orc.iterate("ER", rho, sup, amp, 2, 0.0, 0.0)
+        # five iterations and two trailing projections: HIO amplifies rounding
+        # differences, so the density bar is relative to the FP32 oracle's own
+        # distance from the FP64 oracle (x4); the support must match exactly.
+        rho = orc.iterate("HIO", rho0[i].astype(np.complex128), sup0[i], amp64, 3, 0.9, 0.0)
+        sup = orc.sw(rho, sup0[i], amp64, 2.5, 0.1)
+        rho = orc.iterate("ER", rho, sup, amp64, 2, 0.0, 0.0)
+        r32 = orc.iterate("HIO", rho0[i], sup0[i], amp, 3, 0.9, 0.0)
+        r32 = orc.iterate("ER", r32, sup, amp, 2, 0.0, 0.0)
         assert np.array_equal(r.get_support(), sup)
-        assert rel_l2(r.get_density(), rho) < 1e-5
+        assert rel_l2(r.get_density(), rho) < 4 * max(rel_l2(r32, rho), 1e-6)
+        assert rel_l2(r.get_density(), rho) < 1e-4
         ef, er_, cs, ms = orc.get_error(rho.astype(np.complex128), sup, amp.astype(np.float64), 0.0)
         assert abs(r.get_fourier_error() - ef) <= 1e-5 * ef
         assert abs(r.get_error() - ef) <= 1e-5 * ef          # realpen = 0
