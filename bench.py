@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "individual-iters/s at 512² pop64; time-to-Rf<0.05, batched+grad vs plain GPU"
 ROW_MODE = {"HIO": 2, "RAAR": 3, "ER": 4}       # RowMode of spr_kernels.hpp
+TIMING_STRIDE = 8
 DEVICE_SOURCES = ("spr_fft.hpp", "spr_kernels.hpp", "spr_aux.hpp", "spr_grad.hpp", "spr_backend.hip")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_iteration_512_pop64_raar.json")
 COUNTER_CMD = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 90 rocprofv3 --pmc {ctr} --kernel-trace "
@@ -173,7 +174,7 @@ def measured_traffic(kernel_name, log):
     return None if k is None else k["hbm_bytes"]
 
 
-def time_to_rf_study(seeds, generations, target, log):
+def time_to_rf_study(seeds, generations, target, log, threshold=None):
     """Time to best E_F < target of the plain and the gradient schedules over
     `seeds` seeded 512^2 ellipsoid patterns (population 64); one JSON line."""
     from s_spi_spring_amd import workloads
@@ -182,10 +183,11 @@ def time_to_rf_study(seeds, generations, target, log):
     runs = {"plain": [], "grad": []}
     for seed in range(seeds):
         for form in ("plain", "grad"):
-            r = workloads.time_to_rf(seed, form, target=target, generations=generations)
+            r = workloads.time_to_rf(seed, form, target=target, generations=generations, threshold=threshold)
             runs[form].append(r)
             log(f"seed {seed:2d} {form:5s}: t={r['time_to_target']} s gen={r['generation']} "
-                f"E_F={r['fourier_error']:.4f} real-space={r['real_space_error']:.4f} wall={r['wall']:.1f} s")
+                f"E_F={r['fourier_error']:.4f} (floor {r['noise_floor']:.4f}) real-space={r['real_space_error']:.4f} "
+                f"wall={r['wall']:.1f} s history={r['history']}")
 
     def summary(rs):
         t = [r["time_to_target"] for r in rs if r["time_to_target"] is not None]
@@ -204,8 +206,11 @@ def time_to_rf_study(seeds, generations, target, log):
                                                       "population 64, RAAR400+HIO100+ER50 per generation in "
                                                       "25-iteration blocks with shrink-wrap"},
            "plain": sp, "grad": sg, "speedup_grad_vs_plain": speedup,
+           "sw_threshold": threshold if threshold is not None else workloads.SW_THRESHOLD_512,
+           "median_noise_floor": float(np.median([r["noise_floor"] for r in runs["plain"]])),
            "per_seed": {f: [{k: r[k] for k in ("seed", "time_to_target", "generation", "fourier_error",
-                                               "real_space_error")} for r in rs] for f, rs in runs.items()}}
+                                               "real_space_error", "noise_floor", "history")} for r in rs]
+                        for f, rs in runs.items()}}
     print(json.dumps(out), flush=True)
 
 
@@ -221,6 +226,8 @@ def main():
     ap.add_argument("--beta", type=float, default=0.87)
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-timing", action="store_true",
+                    help="time the steps by wall clock only (no per-launch events, no roofline entry)")
     ap.add_argument("--calibrate", type=int, default=0, metavar="MIB",
                     help="also run device-to-device copies of MIB MiB (counter calibration)")
     ap.add_argument("--time-to-rf", type=int, default=0, metavar="SEEDS",
@@ -228,6 +235,8 @@ def main():
                          "gradient schedules over SEEDS patterns (one JSON line)")
     ap.add_argument("--target", type=float, default=0.05)
     ap.add_argument("--generations", type=int, default=10)
+    ap.add_argument("--sw-threshold", type=float, default=None,
+                    help="shrink-wrap threshold of the time-to-R_F schedules (default 0.03)")
     ap.add_argument("--summarize-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write the HBM traffic summary from two rocprofv3 --pmc runs of "
                          "`bench.py --calibrate MIB` (FETCH_SIZE, WRITE_SIZE) and exit; no GPU needed")
@@ -246,7 +255,7 @@ def main():
         sys.exit(2)
 
     if args.time_to_rf:
-        time_to_rf_study(args.time_to_rf, args.generations, args.target, log)
+        time_to_rf_study(args.time_to_rf, args.generations, args.target, log, args.sw_threshold)
         return
 
     from s_spi_spring_amd import hipcore, synth
@@ -270,7 +279,9 @@ def main():
     be.sync()
 
     log(f"timed: {args.steps} steps of {args.block} {args.alg} iterations, P={P}")
-    be.timing(True)
+    # HIP events bracket every TIMING_STRIDE-th column and row launch of the
+    # timed region (an event pair around every launch would cost ~9 % of it).
+    be.timing(not args.no_timing, stride=TIMING_STRIDE)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         be.iterate_async(args.alg, args.block, args.beta, 0.0)
@@ -282,29 +293,30 @@ def main():
     ind_iters = args.steps * args.block * P
     value = ind_iters / dt
     col_b, row_b = algorithmic_bytes(n, P, args.alg)
-    col_avg = col_ms / max(col_cnt, 1) * 1e-3
-    row_avg = row_ms / max(row_cnt, 1) * 1e-3
-    col_gbs = col_b / col_avg / 1e9
-    row_gbs = row_b / row_avg / 1e9
-    # dominant kernel = the one with the larger share of the timed region
-    if row_ms >= col_ms:
-        dom = {"kernel": rocprof_kernel_name("row", n, args.alg), "achieved": row_gbs, "bytes_per_launch": row_b,
-               "avg_launch_us": row_avg * 1e6}
-    else:
-        dom = {"kernel": rocprof_kernel_name("col", n, args.alg), "achieved": col_gbs, "bytes_per_launch": col_b,
-               "avg_launch_us": col_avg * 1e6}
-    log(f"column pass: {col_cnt} launches, avg {col_avg * 1e6:.1f} us, {col_gbs:.0f} GB/s algorithmic")
-    log(f"row pass:    {row_cnt} launches, avg {row_avg * 1e6:.1f} us, {row_gbs:.0f} GB/s algorithmic")
     iter_bytes = (col_b + row_b) / P
-    log(f"whole iteration: {value * iter_bytes / 1e9:.0f} GB/s algorithmic "
-        f"({value * iter_bytes / 1e9 / HBM_PEAK_GBS:.3f} of HBM peak)")
-
-    roofline = {"bound": "hbm", "achieved": round(dom["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dom["achieved"] / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(dom["kernel"], log),
-                "kernel": dom["kernel"], "algorithmic_bytes_per_launch": dom["bytes_per_launch"],
-                "avg_launch_us": round(dom["avg_launch_us"], 2),
-                "iteration_frac": round(value * iter_bytes / 1e9 / HBM_PEAK_GBS, 4)}
+    iteration_frac = value * iter_bytes / 1e9 / HBM_PEAK_GBS
+    log(f"whole iteration: {value * iter_bytes / 1e9:.0f} GB/s algorithmic ({iteration_frac:.3f} of HBM peak)")
+    roofline = None
+    if col_cnt and row_cnt:
+        col_avg = col_ms / col_cnt * 1e-3
+        row_avg = row_ms / row_cnt * 1e-3
+        col_gbs = col_b / col_avg / 1e9
+        row_gbs = row_b / row_avg / 1e9
+        # dominant kernel = the one with the larger share of the timed region
+        if row_ms >= col_ms:
+            dom = {"kernel": rocprof_kernel_name("row", n, args.alg), "achieved": row_gbs,
+                   "bytes_per_launch": row_b, "avg_launch_us": row_avg * 1e6}
+        else:
+            dom = {"kernel": rocprof_kernel_name("col", n, args.alg), "achieved": col_gbs,
+                   "bytes_per_launch": col_b, "avg_launch_us": col_avg * 1e6}
+        log(f"column pass: {col_cnt} launches, avg {col_avg * 1e6:.1f} us, {col_gbs:.0f} GB/s algorithmic")
+        log(f"row pass:    {row_cnt} launches, avg {row_avg * 1e6:.1f} us, {row_gbs:.0f} GB/s algorithmic")
+        roofline = {"bound": "hbm", "achieved": round(dom["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(dom["achieved"] / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(dom["kernel"], log),
+                    "kernel": dom["kernel"], "algorithmic_bytes_per_launch": dom["bytes_per_launch"],
+                    "avg_launch_us": round(dom["avg_launch_us"], 2),
+                    "iteration_frac": round(iteration_frac, 4)}
 
     cpu = None
     if not args.no_cpu:

@@ -89,6 +89,28 @@ int spr_set_population(spr_ctx* ctx, int P, const float* rho, const int32_t* sup
  * may be NULL. */
 int spr_get_population(spr_ctx* ctx, int P, float* rho, int32_t* support);
 
+/* Slot-level residency: the handle holds `capacity` slots; operations act on
+ * the active slots [0, P). These replace the per-individual set_density /
+ * set_support / get_density / get_support of one Context per OpenMP thread
+ * (src/context.h:23-26, src/solver.cpp:360-411) so that a caller can keep
+ * individuals resident in HBM between operations instead of copying every
+ * individual in and out around each one.
+ *
+ * spr_set_active: the active population size P (no data moves).
+ * spr_gather: slot i <- slot src[i] for 0 <= i < count (density and support);
+ *   src[i] = -1 or i leaves slot i alone; all sources are read before any slot
+ *   is written, so src may be any mapping.
+ * spr_put_individuals: upload individual k to slot slots[k] from separate host
+ *   arrays (rho[k]: n*n complex, support[k]: n*n int32); either pointer array,
+ *   or an entry of rho, may be NULL to leave that part.
+ * spr_fetch_individuals: download slot slots[k] into rho[k] / support[k]
+ *   (NULL arrays or entries are skipped). */
+int spr_set_active(spr_ctx* ctx, int P);
+int spr_gather(spr_ctx* ctx, int count, const int* src);
+int spr_put_individuals(spr_ctx* ctx, int count, const int* slots, const float* const* rho,
+                        const int32_t* const* support);
+int spr_fetch_individuals(spr_ctx* ctx, int count, const int* slots, float* const* rho, int32_t* const* support);
+
 /* Context::get_ft / get_ift (src/context.h:21-22, src/gpu/gpu_context.cu:748-764)
  * applied to every resident individual's density. */
 int spr_ft(spr_ctx* ctx);
@@ -205,12 +227,16 @@ int spr_loss_grad(spr_ctx* ctx, int P, const float* rho, float* grad, double* lo
 int spr_refine(spr_ctx* ctx, int steps, float lr, int kind, float eps, double* loss);
 
 /* Device addresses of the resident population: rho (P*n*n float2) and the
- * uint8 supports; for callers that fill or read HBM directly. */
+ * uint8 supports; for callers that fill or read HBM directly. The amplitude
+ * (n*n float) is for reading only: the context also keeps a transposed copy,
+ * so change it through spr_set_pattern. */
 int spr_device_buffers(spr_ctx* ctx, void** rho, void** support_u8, void** amp);
 
-/* Opt-in kernel timing. With timing enabled, every launch of the handle is
- * bracketed by HIP events on its stream; after each synchronising call the
- * durations are added per kind. spr_timing() also resets the totals. */
+/* Opt-in kernel timing. spr_timing(ctx, k) with k >= 1 brackets every k-th
+ * launch of each kind by HIP events on the handle's stream (k = 1: every
+ * launch; a larger k keeps the events' own cost out of the timed work); after
+ * each synchronising call the durations are added per kind. k = 0 turns timing
+ * off. spr_timing() also resets the totals. */
 enum spr_timing_kind {
     SPR_KIND_COL_PROJ = 0,     /* column pass colIFFT(P_M(colFFT)) */
     SPR_KIND_ROW_UPDATE = 1,   /* row pass rowIFFT -> HIO/RAAR/ER update -> rowFFT */

@@ -4,12 +4,19 @@
 #pragma once
 
 #include <array>
+#include <atomic>
 #include <complex>
 #include <cstdint>
 #include <cstring>
 #include <vector>
 
 namespace spring {
+
+// Process-wide counter for Reconstruction::version.
+inline uint64_t next_version() {
+    static std::atomic<uint64_t> v{1};
+    return v.fetch_add(1, std::memory_order_relaxed);
+}
 
 using FLOAT = float;
 using COMPLEX = std::complex<float>;
@@ -37,8 +44,14 @@ public:
 
     std::vector<COMPLEX> get_density() const { return density; }
     std::vector<SUPPORTTYPE> get_support() const { return support; }
-    void set_support(const SUPPORTTYPE* s) { std::memcpy(support.data(), s, support.size() * sizeof(SUPPORTTYPE)); }
-    void set_density(const COMPLEX* d) { std::memcpy(density.data(), d, density.size() * sizeof(COMPLEX)); }
+    void set_support(const SUPPORTTYPE* s) {
+        std::memcpy(support.data(), s, support.size() * sizeof(SUPPORTTYPE));
+        version = next_version();
+    }
+    void set_density(const COMPLEX* d) {
+        std::memcpy(density.data(), d, density.size() * sizeof(COMPLEX));
+        version = next_version();
+    }
 
     FLOAT get_error() const { return error; }
     FLOAT get_fourier_error() const { return error_fourier; }
@@ -58,6 +71,9 @@ public:
     FLOAT modulesum = 0;
     FLOAT complexsum = 0;
     FLOAT symmetry = 0;
+    // Changes whenever density or support change (a device copy of this
+    // individual is current only while it carries the same version).
+    uint64_t version = next_version();
 };
 
 }  // namespace spring

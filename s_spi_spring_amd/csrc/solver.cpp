@@ -11,6 +11,8 @@
 #include <numeric>
 #include <stdexcept>
 #include <thread>
+#include <unordered_map>
+#include <unordered_set>
 
 namespace spring {
 
@@ -25,6 +27,21 @@ FLOAT field_phase(const std::vector<COMPLEX>& d) {
         sy += z.imag();
     }
     return std::atan2(sy, sx);
+}
+
+// f(i) for i in [0, n) on up to 16 host threads (independent iterations).
+template <class F> void parallel_for(int n, F f) {
+    const int nt = std::min(n, (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency())));
+    if (nt <= 1) {
+        for (int i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&, t] {
+            for (int i = t; i < n; i += nt) f(i);
+        });
+    for (std::thread& th : pool) th.join();
 }
 
 }  // namespace
@@ -147,41 +164,109 @@ spr_ctx* Solver::backend(int capacity) {
         throw std::runtime_error(std::string("spring_hip: ") + spr_last_error(nullptr));
     ctx = c;
     ctx_capacity = cap;
+    slots.assign(cap, Slot{});
     return ctx;
 }
 
-void Solver::upload(const RecVec& rec, bool support) {
-    const int P = (int)rec.size();
-    backend(P);
-    std::vector<COMPLEX> rho((std::size_t)P * size);
-    std::vector<SUPPORTTYPE> s(support ? (std::size_t)P * size : 0);
-    for (int i = 0; i < P; ++i) {
-        if (rec[i]->density.size() != size) throw std::invalid_argument("reconstruction size differs from the pattern");
-        std::copy(rec[i]->density.begin(), rec[i]->density.end(), rho.begin() + (std::size_t)i * size);
-        if (support) std::copy(rec[i]->support.begin(), rec[i]->support.end(), s.begin() + (std::size_t)i * size);
-    }
-    check(spr_set_population(ctx, P, reinterpret_cast<const float*>(rho.data()), support ? s.data() : nullptr));
+void Solver::forget(int count) {
+    for (int j = 0; j < count && j < (int)slots.size(); ++j) slots[j] = Slot{};
 }
 
-void Solver::download(const RecVec& rec, bool support) {
+// Make rec[i] the content of slot i (i < P) and P the active size. Individuals
+// whose current version is resident are gathered on the device; the others are
+// copied in from the host. Other resident individuals displaced from [0, P) are
+// parked in free slots >= P while there is room.
+void Solver::place(const RecVec& rec) {
     const int P = (int)rec.size();
-    std::vector<COMPLEX> rho((std::size_t)P * size);
-    std::vector<SUPPORTTYPE> s(support ? (std::size_t)P * size : 0);
-    check(spr_get_population(ctx, P, reinterpret_cast<float*>(rho.data()), support ? s.data() : nullptr));
-    for (int i = 0; i < P; ++i) {
-        std::copy(rho.begin() + (std::size_t)i * size, rho.begin() + (std::size_t)(i + 1) * size,
-                  rec[i]->density.begin());
-        if (support)
-            std::copy(s.begin() + (std::size_t)i * size, s.begin() + (std::size_t)(i + 1) * size,
-                      rec[i]->support.begin());
+    backend(2 * P + 2);
+    const int cap = ctx_capacity;
+    std::unordered_map<const Reconstruction*, int> where;
+    std::vector<RecPtr> owner(cap);
+    for (int j = 0; j < cap; ++j) {
+        RecPtr o = slots[j].owner.lock();
+        if (o && o->version == slots[j].version) {
+            owner[j] = o;
+            where.emplace(o.get(), j);
+        } else {
+            slots[j] = Slot{};
+        }
     }
+    std::unordered_set<const Reconstruction*> wanted;
+    for (const RecPtr& r : rec) {
+        if (r->density.size() != size || r->support.size() != size)
+            throw std::invalid_argument("reconstruction size differs from the pattern");
+        wanted.insert(r.get());
+    }
+    std::vector<int> src(cap, -1);
+    std::vector<Slot> next(cap);
+    std::vector<char> taken(cap, 0);
+    std::vector<int> up;
+    for (int i = 0; i < P; ++i) {
+        auto it = where.find(rec[i].get());
+        if (it != where.end()) src[i] = it->second;
+        else up.push_back(i);
+        next[i] = Slot{rec[i], rec[i]->version};
+        taken[i] = 1;
+    }
+    for (int j = P; j < cap; ++j)
+        if (owner[j] && !wanted.count(owner[j].get())) {
+            next[j] = slots[j];
+            taken[j] = 1;
+        }
+    int f = P;
+    for (int j = 0; j < P && j < cap; ++j) {
+        if (!owner[j] || wanted.count(owner[j].get())) continue;
+        while (f < cap && taken[f]) ++f;
+        if (f >= cap) break;
+        src[f] = j;
+        next[f] = slots[j];
+        taken[f] = 1;
+    }
+    // Slots >= P that neither move nor receive keep their (still current) content.
+    for (int j = P; j < cap; ++j)
+        if (!taken[j] && src[j] < 0 && owner[j]) next[j] = slots[j];
+    check(spr_gather(ctx, cap, src.data()));
+    slots = next;
+    if (!up.empty()) {
+        std::vector<const float*> rp(up.size());
+        std::vector<const int32_t*> sp(up.size());
+        for (std::size_t k = 0; k < up.size(); ++k) {
+            rp[k] = reinterpret_cast<const float*>(rec[up[k]]->density.data());
+            sp[k] = rec[up[k]]->support.data();
+        }
+        check(spr_put_individuals(ctx, (int)up.size(), up.data(), rp.data(), sp.data()));
+    }
+    check(spr_set_active(ctx, P));
+}
+
+void Solver::pull(const RecVec& rec, bool density, bool support) {
+    const int P = (int)rec.size();
+    if (P == 0 || (!density && !support)) return;
+    std::vector<int> idx(P);
+    std::vector<float*> rp(P);
+    std::vector<int32_t*> sp(P);
+    for (int i = 0; i < P; ++i) {
+        idx[i] = i;
+        rp[i] = reinterpret_cast<float*>(rec[i]->density.data());
+        sp[i] = rec[i]->support.data();
+    }
+    check(spr_fetch_individuals(ctx, P, idx.data(), density ? rp.data() : nullptr, support ? sp.data() : nullptr));
+    for (int i = 0; i < P; ++i) {
+        rec[i]->version = next_version();
+        slots[i] = Slot{rec[i], rec[i]->version};
+    }
+    // An object listed twice holds the last slot's data; its other slot is stale.
+    for (int i = 0; i < P; ++i)
+        for (int j = i + 1; j < P; ++j)
+            if (rec[i] == rec[j]) slots[i] = Slot{};
 }
 
 void Solver::run_op(const RecVec& rec, const spr_op& op) {
     if (rec.empty()) return;
-    upload(rec);
+    place(rec);
     check(spr_run(ctx, &op, 1, 1, reality));
-    download(rec);
+    const bool support_only = op.kind == SPR_OP_SW || op.kind == SPR_OP_SWA;
+    pull(rec, !support_only, true);
 }
 
 // Solver::HIO / RAAR / ER / SW / SWA (src/solver.cpp:360-450).
@@ -257,17 +342,17 @@ RecVec Solver::Sequence(RecVec rec, int nrep) {
         return rec;
     }
     if (rec.empty()) return rec;
-    upload(rec);
+    place(rec);
     check(spr_run(ctx, ops.data(), (int)ops.size(), nrep, reality));
     evaluate_resident(rec);
-    download(rec);
+    pull(rec, true, true);
     return rec;
 }
 
 // Solver::Evaluate (src/solver.cpp:541-558).
 void Solver::Evaluate(RecVec rec) {
     if (rec.empty()) return;
-    upload(rec);
+    place(rec);
     evaluate_resident(rec);
 }
 
@@ -275,7 +360,7 @@ void Solver::Evaluate(RecVec rec) {
 RecVec Solver::Reshift(RecVec rec, RecPtr ref) {
     const int P = (int)rec.size();
     if (P == 0) return rec;
-    upload(rec);
+    place(rec);
     const FLOAT ref_phase = field_phase(ref->density);
     std::vector<float> direct(P), flip(P);
     for (int i = 0; i < P; ++i) {
@@ -285,7 +370,7 @@ RecVec Solver::Reshift(RecVec rec, RecPtr ref) {
     }
     check(spr_align(ctx, reinterpret_cast<const float*>(ref->density.data()), shiftmin, shiftmax, direct.data(),
                     flip.data(), nullptr));
-    download(rec);
+    pull(rec, true, true);
     return rec;
 }
 
@@ -304,7 +389,7 @@ RecPtr Solver::GetAverage(RecVec rec, FLOAT fraction) {
     navg = std::min(navg, P);
     navg = std::max(navg, 1);
     const std::vector<int> idx = sorted_by_error(rec);
-    upload(rec);
+    place(rec);
     std::vector<COMPLEX> d(size);
     std::vector<SUPPORTTYPE> s(size);
     check(spr_average(ctx, idx.data(), navg, reinterpret_cast<float*>(d.data()), s.data()));
@@ -319,7 +404,7 @@ std::vector<FLOAT> Solver::GetAverageModule(RecVec rec, bool ft, FLOAT fraction)
     navg = std::min(navg, P);
     navg = std::max(navg, 1);
     const std::vector<int> idx = sorted_by_error(rec);
-    upload(rec, false);
+    place(rec);
     std::vector<FLOAT> out(size);
     check(spr_average_module(ctx, idx.data(), navg, ft ? 1 : 0, out.data()));
     return out;
@@ -379,7 +464,7 @@ RecVec Solver::Crossover(RecVec rec, RecPtr best, RecPtr avg, FLOAT crossbalance
             d.phase[2 * k + 1] = p1ph + role_phase[k];
         }
     }
-    upload(rec);
+    place(rec);
     std::vector<COMPLEX> child((std::size_t)P * size);
     std::vector<SUPPORTTYPE> child_s((std::size_t)P * size);
     check(spr_crossover(ctx, P, desc.data(), maps.data(), (int)maps.size(),
@@ -411,7 +496,8 @@ void Solver::set_pattern(const FLOAT* pattern_in) {
 // Batched 2-D transforms of host fields on the device (Context::get_ft / get_ift).
 std::vector<COMPLEX> Solver::ft_batch(const std::vector<std::vector<COMPLEX>>& fields, bool inverse) {
     const int P = (int)fields.size();
-    backend(P);
+    backend(std::max(2 * P + 2, ctx_capacity));
+    forget(P);
     std::vector<COMPLEX> buf((std::size_t)P * size);
     for (int i = 0; i < P; ++i) std::copy(fields[i].begin(), fields[i].end(), buf.begin() + (std::size_t)i * size);
     check(spr_set_population(ctx, P, reinterpret_cast<const float*>(buf.data()), nullptr));
@@ -429,21 +515,30 @@ RecVec Solver::Initialize_pop(int nrec, int support_size, std::array<FLOAT, 2> i
     const int n0 = (int)dims[0], n1 = (int)dims[1];
     std::vector<std::vector<COMPLEX>> dens(nrec, std::vector<COMPLEX>(size));
     std::vector<std::vector<SUPPORTTYPE>> supp(nrec, std::vector<SUPPORTTYPE>(size, 0));
-    std::vector<FLOAT> scale(nrec);
+    std::vector<FLOAT> scale(nrec), expo(nrec);
+    std::vector<std::vector<FLOAT>> sig(nrec), cx(nrec), cy(nrec), ph(nrec);
+    // The draws of individual r come from stream r % nthreads in the reference's
+    // order; the fields are then built from them independently.
     for (int r = 0; r < nrec; ++r) {
         ContextRng& g = rngs[r % nthreads];
         const int nit = (int)g.rand(nitems_in[0], nitems_in[1] + 0.99);
-        const FLOAT expo = g.rand(exponent_in[0], exponent_in[1]);
+        expo[r] = g.rand(exponent_in[0], exponent_in[1]);
         scale[r] = g.rand(scaling_in[0], scaling_in[1]);
-        std::vector<FLOAT> sig(nit), cx(nit), cy(nit), ph(nit);
+        sig[r].resize(nit);
+        cx[r].resize(nit);
+        cy[r].resize(nit);
+        ph[r].resize(nit);
         for (int i = 0; i < nit; ++i) {
             const FLOAT s = std::sqrt(g.rand(std::pow(itemsize[0], 2), std::pow(itemsize[1], 2))) / 2.;
-            cx[i] = g.rand(s, support_size - s);
-            cy[i] = g.rand(s, support_size - s);
-            sig[i] = s;
-            ph[i] = g.rand(-phase_in * M_PI, phase_in * M_PI);
+            cx[r][i] = g.rand(s, support_size - s);
+            cy[r][i] = g.rand(s, support_size - s);
+            sig[r][i] = s;
+            ph[r][i] = g.rand(-phase_in * M_PI, phase_in * M_PI);
         }
+    }
+    parallel_for(nrec, [&](int r) {
         std::vector<COMPLEX>& d = dens[r];
+        const int nit = (int)sig[r].size();
         for (int iy = 0; iy < support_size; ++iy)
             for (int ix = 0; ix < support_size; ++ix) {
                 const double dx = ix - support_size / 2., dy = iy - support_size / 2.;
@@ -451,18 +546,18 @@ RecVec Solver::Initialize_pop(int nrec, int support_size, std::array<FLOAT, 2> i
                 const int idx = (iy + n1 / 2 - support_size / 2) * n0 + (ix + n0 / 2 - support_size / 2);
                 supp[r][idx] = 1;
                 for (int i = 0; i < nit; ++i) {
-                    const FLOAT rad = std::sqrt((ix - cx[i]) * (ix - cx[i]) + (iy - cy[i]) * (iy - cy[i]));
-                    if (rad < sig[i]) {
-                        const FLOAT val = 2. * sig[i] * std::sin(std::acos(rad / sig[i]));
-                        d[idx] += std::polar(val, ph[i]);
+                    const FLOAT rad = std::sqrt((ix - cx[r][i]) * (ix - cx[r][i]) + (iy - cy[r][i]) * (iy - cy[r][i]));
+                    if (rad < sig[r][i]) {
+                        const FLOAT val = 2. * sig[r][i] * std::sin(std::acos(rad / sig[r][i]));
+                        d[idx] += std::polar(val, ph[r][i]);
                     }
                 }
                 const FLOAT mod = std::abs(d[idx]);
-                if (mod > 0) d[idx] = d[idx] / mod * std::pow(mod, expo);
+                if (mod > 0) d[idx] = d[idx] / mod * std::pow(mod, expo[r]);
             }
-    }
+    });
     const std::vector<COMPLEX> F = ft_batch(dens, false);
-    for (int r = 0; r < nrec; ++r) {
+    parallel_for(nrec, [&](int r) {
         FLOAT norm = 0, norm_new = 0;
         for (std::size_t i = 0; i < size; ++i)
             if (pattern[i] >= 0) {
@@ -472,7 +567,7 @@ RecVec Solver::Initialize_pop(int nrec, int support_size, std::array<FLOAT, 2> i
         const FLOAT f = std::sqrt(norm_new / norm) * scale[r];
         for (COMPLEX& z : dens[r]) z *= f;
         rec[r] = std::make_shared<Reconstruction>(dens[r].data(), supp[r].data(), dims);
-    }
+    });
     return rec;
 }
 

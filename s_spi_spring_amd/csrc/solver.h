@@ -85,8 +85,17 @@ public:
 
 private:
     spr_ctx* backend(int capacity);
-    void upload(const RecVec& rec, bool support = true);
-    void download(const RecVec& rec, bool support = true);
+    // Device residency. The backend holds 2P+2 slots; an individual that is
+    // already resident (same object, same version) is moved between slots on
+    // the device instead of being copied in again. Host copies stay current:
+    // every operation that changes individuals copies the changes back.
+    struct Slot {
+        std::weak_ptr<Reconstruction> owner;
+        uint64_t version = 0;
+    };
+    void place(const RecVec& rec);                           // rec[i] -> slot i, active P = rec.size()
+    void pull(const RecVec& rec, bool density, bool support);  // slots [0, P) -> rec, now current
+    void forget(int count);                                  // slots [0, count) no longer hold individuals
     void run_op(const RecVec& rec, const spr_op& op);
     void evaluate_resident(const RecVec& rec);
     void check(int rc) const;
@@ -97,6 +106,7 @@ private:
     int device;
     spr_ctx* ctx = nullptr;
     int ctx_capacity = 0;
+    std::vector<Slot> slots;
 };
 
 }  // namespace spring

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -17,11 +18,19 @@ using namespace spr;
 struct spr_ctx {
     int n = 0, cap = 0, P = 0, device = 0;
     hipStream_t stream = nullptr;
+    // Second lane of the pipelined iteration: the population is split in two
+    // halves whose column and row passes run on two streams half an iteration
+    // apart, so that the VALU-bound column pass of one half overlaps the
+    // HBM-bound row pass of the other.
+    int lanes = 1;
+    hipStream_t lane2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     float2* rho = nullptr;
     float2* work = nullptr;
     float2* work2 = nullptr;      // second half-transformed field per individual (correlations, |b|)
     uint8_t* supp = nullptr;
     float* amp = nullptr;
+    float* amp_t = nullptr;       // amp transposed (read along columns by the column pass)
     float2* tw = nullptr;
     // reduction scratch
     double* partial = nullptr;    // per workgroup partial sums
@@ -36,10 +45,13 @@ struct spr_ctx {
         void* p = nullptr;
         size_t bytes = 0;
     };
-    Buf spec, xf, xs, supp2, fieldbuf, ampavg, xidx, shiftp, xchild, xmaps, xphase, adam_m, adam_v;
+    Buf spec, xf, xs, supp2, fieldbuf, ampavg, xidx, shiftp, xchild, xmaps, xphase, adam_m, adam_v, gidx, sstage;
     std::string err;
     // Opt-in per-launch timing: events bracket every launch on the stream.
     bool timing = false;
+    int timing_stride = 1;             // bracket every timing_stride-th launch of each kind
+    long kind_seen[SPR_TIMING_KINDS] = {};
+    bool bracket_open = false;
     std::vector<hipEvent_t> ev_pool;
     std::vector<int> ev_kind;          // kind of the launch bracketed by events 2i, 2i+1
     double kind_ms[SPR_TIMING_KINDS] = {};
@@ -78,7 +90,10 @@ float next_down(double x) {
 
 // Event bracket of one launch (no-op unless timing is enabled).
 int timing_begin(spr_ctx* c, int kind) {
+    c->bracket_open = false;
     if (!c->timing) return SPR_OK;
+    if (c->kind_seen[kind]++ % c->timing_stride != 0) return SPR_OK;
+    c->bracket_open = true;
     const size_t i = c->ev_kind.size();
     while (c->ev_pool.size() < 2 * (i + 1)) {
         hipEvent_t e;
@@ -90,7 +105,8 @@ int timing_begin(spr_ctx* c, int kind) {
     return SPR_OK;
 }
 int timing_end(spr_ctx* c) {
-    if (!c->timing) return SPR_OK;
+    if (!c->bracket_open) return SPR_OK;
+    c->bracket_open = false;
     const size_t i = c->ev_kind.size() - 1;
     SPR_HIP(c, hipEventRecord(c->ev_pool[2 * i + 1], c->stream));
     return SPR_OK;
@@ -106,6 +122,13 @@ int harvest_timing(spr_ctx* c) {
     return SPR_OK;
 }
 
+std::vector<float> transpose_amp(const float* amp, int n) {
+    std::vector<float> t((size_t)n * n);
+    for (int y = 0; y < n; ++y)
+        for (int x = 0; x < n; ++x) t[(size_t)x * n + y] = amp[(size_t)y * n + x];
+    return t;
+}
+
 // Reality windows of src/gpu/gpu_context.cu:530, 553 (ER/HIO) and 582 (RAAR).
 PhaseCut cut_er_hio(float r) {
     const double t = (double)(1.f - r) * M_PI;
@@ -115,29 +138,31 @@ PhaseCut cut_raar(float r) {
     return PhaseCut{next_up((1.0 - (double)r) * M_PI), next_down(-(double)(1.f - r) * M_PI), r == 0.f ? 1 : 0};
 }
 
-template <int N> int launch_row(spr_ctx* c, int mode, const RowArgs& a) {
-    const dim3 grid((unsigned)((size_t)c->P * N / Geo<N>::RW)), block(Geo<N>::ROW_THREADS);
+// Row / column passes over `cnt` individuals starting at the pointers in the
+// arguments, on stream `s`.
+template <int N> int launch_row(spr_ctx* c, int mode, const RowArgs& a, int cnt, hipStream_t s) {
+    const dim3 grid((unsigned)((size_t)cnt * N / Geo<N>::RW)), block(Geo<N>::ROW_THREADS);
     switch (mode) {
-        case ROW_FWD: hipLaunchKernelGGL((k_row<N, ROW_FWD>), grid, block, 0, c->stream, a); break;
-        case ROW_FINAL: hipLaunchKernelGGL((k_row<N, ROW_FINAL>), grid, block, 0, c->stream, a); break;
-        case ROW_HIO: hipLaunchKernelGGL((k_row<N, ROW_HIO>), grid, block, 0, c->stream, a); break;
-        case ROW_RAAR: hipLaunchKernelGGL((k_row<N, ROW_RAAR>), grid, block, 0, c->stream, a); break;
-        case ROW_ER: hipLaunchKernelGGL((k_row<N, ROW_ER>), grid, block, 0, c->stream, a); break;
-        case ROW_FWD_INPLACE: hipLaunchKernelGGL((k_row<N, ROW_FWD_INPLACE>), grid, block, 0, c->stream, a); break;
-        case ROW_INV_INPLACE: hipLaunchKernelGGL((k_row<N, ROW_INV_INPLACE>), grid, block, 0, c->stream, a); break;
+        case ROW_FWD: hipLaunchKernelGGL((k_row<N, ROW_FWD>), grid, block, 0, s, a); break;
+        case ROW_FINAL: hipLaunchKernelGGL((k_row<N, ROW_FINAL>), grid, block, 0, s, a); break;
+        case ROW_HIO: hipLaunchKernelGGL((k_row<N, ROW_HIO>), grid, block, 0, s, a); break;
+        case ROW_RAAR: hipLaunchKernelGGL((k_row<N, ROW_RAAR>), grid, block, 0, s, a); break;
+        case ROW_ER: hipLaunchKernelGGL((k_row<N, ROW_ER>), grid, block, 0, s, a); break;
+        case ROW_FWD_INPLACE: hipLaunchKernelGGL((k_row<N, ROW_FWD_INPLACE>), grid, block, 0, s, a); break;
+        case ROW_INV_INPLACE: hipLaunchKernelGGL((k_row<N, ROW_INV_INPLACE>), grid, block, 0, s, a); break;
         default: return fail(c, SPR_ERR_ARG, "bad row mode");
     }
     SPR_HIP(c, hipGetLastError());
     return SPR_OK;
 }
 
-template <int N> int launch_col(spr_ctx* c, int mode, float2* f) {
-    const ColArgs a{f, c->amp, c->tw};
-    const dim3 grid((unsigned)((size_t)c->P * (N / Geo<N>::C))), block(Geo<N>::COL_THREADS);
+template <int N> int launch_col(spr_ctx* c, int mode, float2* f, int cnt, hipStream_t s) {
+    const ColArgs a{f, c->amp, c->tw, c->amp_t};
+    const dim3 grid((unsigned)((size_t)cnt * (N / Geo<N>::C))), block(Geo<N>::COL_THREADS);
     switch (mode) {
-        case COL_PROJ: hipLaunchKernelGGL((k_col<N, COL_PROJ>), grid, block, 0, c->stream, a); break;
-        case COL_FWD: hipLaunchKernelGGL((k_col<N, COL_FWD>), grid, block, 0, c->stream, a); break;
-        case COL_INV: hipLaunchKernelGGL((k_col<N, COL_INV>), grid, block, 0, c->stream, a); break;
+        case COL_PROJ: hipLaunchKernelGGL((k_col<N, COL_PROJ>), grid, block, 0, s, a); break;
+        case COL_FWD: hipLaunchKernelGGL((k_col<N, COL_FWD>), grid, block, 0, s, a); break;
+        case COL_INV: hipLaunchKernelGGL((k_col<N, COL_INV>), grid, block, 0, s, a); break;
         default: return fail(c, SPR_ERR_ARG, "bad column mode");
     }
     SPR_HIP(c, hipGetLastError());
@@ -155,14 +180,16 @@ template <int N> int launch_col(spr_ctx* c, int mode, float2* f) {
         default: return fail((c), SPR_ERR_ARG, "unsupported grid size"); \
     }
 
-int row_n(spr_ctx* c, int mode, const RowArgs& a) { SPR_DISPATCH_N(c, launch_row, c, mode, a); }
-int col_n(spr_ctx* c, int mode, float2* f) { SPR_DISPATCH_N(c, launch_col, c, mode, f); }
+int row_n(spr_ctx* c, int mode, const RowArgs& a, int cnt, hipStream_t s) {
+    SPR_DISPATCH_N(c, launch_row, c, mode, a, cnt, s);
+}
+int col_n(spr_ctx* c, int mode, float2* f, int cnt, hipStream_t s) { SPR_DISPATCH_N(c, launch_col, c, mode, f, cnt, s); }
 
 int row(spr_ctx* c, int mode, const RowArgs& a) {
     const int kind = (mode == ROW_HIO || mode == ROW_RAAR || mode == ROW_ER) ? SPR_KIND_ROW_UPDATE : SPR_KIND_ROW_OTHER;
     int rc = timing_begin(c, kind);
     if (rc != SPR_OK) return rc;
-    rc = row_n(c, mode, a);
+    rc = row_n(c, mode, a, c->P, c->stream);
     if (rc != SPR_OK) return rc;
     return timing_end(c);
 }
@@ -170,7 +197,7 @@ int col(spr_ctx* c, int mode, float2* f) {
     const int kind = (mode == COL_PROJ) ? SPR_KIND_COL_PROJ : SPR_KIND_COL_OTHER;
     int rc = timing_begin(c, kind);
     if (rc != SPR_OK) return rc;
-    rc = col_n(c, mode, f);
+    rc = col_n(c, mode, f, c->P, c->stream);
     if (rc != SPR_OK) return rc;
     return timing_end(c);
 }
@@ -409,6 +436,39 @@ float key_value(unsigned long long k) {
 }
 int key_index(unsigned long long k) { return k ? (int)(0xFFFFFFFFu - (unsigned)(k & 0xFFFFFFFFu)) : 0; }
 
+// Pipelined form of the iteration over two halves of the population: lane 1
+// (individuals [0, h), the context's stream) and lane 2 ([h, P), a second
+// stream) issue the same pass sequence; lane 2 starts after lane 1's first row
+// pass, so that the column pass of each half runs beside the row pass of the
+// other. The halves share no memory (the twiddles and the amplitude are read
+// only); the context's stream waits for lane 2 at the end.
+int queue_iterate_lanes(spr_ctx* c, int mode, int niter, const RowArgs& a) {
+    const int h = (c->P + 1) / 2, h2 = c->P - h;
+    const size_t off = (size_t)h * c->n * c->n;
+    RowArgs b = a;
+    b.rho += off;
+    b.work += off;
+    b.supp += off;
+    hipStream_t s1 = c->stream, s2 = c->lane2;
+    SPR_TRY(row_n(c, ROW_FWD, a, h, s1));
+    SPR_HIP(c, hipEventRecord(c->ev_fork, s1));
+    SPR_HIP(c, hipStreamWaitEvent(s2, c->ev_fork, 0));
+    SPR_TRY(row_n(c, ROW_FWD, b, h2, s2));
+    for (int i = 0; i < niter; ++i) {
+        SPR_TRY(col_n(c, COL_PROJ, a.work, h, s1));
+        SPR_TRY(col_n(c, COL_PROJ, b.work, h2, s2));
+        SPR_TRY(row_n(c, mode, a, h, s1));
+        SPR_TRY(row_n(c, mode, b, h2, s2));
+    }
+    SPR_TRY(col_n(c, COL_PROJ, a.work, h, s1));
+    SPR_TRY(col_n(c, COL_PROJ, b.work, h2, s2));
+    SPR_TRY(row_n(c, ROW_FINAL, a, h, s1));
+    SPR_TRY(row_n(c, ROW_FINAL, b, h2, s2));
+    SPR_HIP(c, hipEventRecord(c->ev_join, s2));
+    SPR_HIP(c, hipStreamWaitEvent(s1, c->ev_join, 0));
+    return SPR_OK;
+}
+
 int queue_iterate(spr_ctx* c, int alg, int niter, float beta, float reality) {
     if (alg < SPR_ER || alg > SPR_RAAR) return fail(c, SPR_ERR_ARG, "unknown algorithm");
     if (niter < 0) return fail(c, SPR_ERR_ARG, "negative iteration count");
@@ -418,6 +478,7 @@ int queue_iterate(spr_ctx* c, int alg, int niter, float beta, float reality) {
     a.pc = (alg == SPR_RAAR) ? cut_raar(reality) : cut_er_hio(reality);
     const int mode = (alg == SPR_HIO) ? ROW_HIO : (alg == SPR_RAAR) ? ROW_RAAR : ROW_ER;
     SPR_HIP(c, hipSetDevice(c->device));
+    if (c->lanes > 1 && c->P >= 2 && !c->timing) return queue_iterate_lanes(c, mode, niter, a);
     SPR_TRY(row(c, ROW_FWD, a));
     for (int i = 0; i < niter; ++i) {
         SPR_TRY(col(c, COL_PROJ, c->work));
@@ -600,11 +661,20 @@ int spr_create(spr_ctx** out, int n, int capacity, const float* amp, int device)
     if ((e = hipSetDevice(device)) != hipSuccess) return cleanup_fail("hipSetDevice", e);
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
         return cleanup_fail("hipStreamCreate", e);
+    if (const char* l = std::getenv("SPR_ITERATE_LANES")) c->lanes = std::atoi(l) == 2 ? 2 : 1;
+    if (c->lanes == 2) {
+        if ((e = hipStreamCreateWithFlags(&c->lane2, hipStreamNonBlocking)) != hipSuccess)
+            return cleanup_fail("hipStreamCreate", e);
+        if ((e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming)) != hipSuccess ||
+            (e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming)) != hipSuccess)
+            return cleanup_fail("hipEventCreate", e);
+    }
     if ((e = hipMalloc(&c->rho, np * capacity * sizeof(float2))) != hipSuccess) return cleanup_fail("hipMalloc rho", e);
     if ((e = hipMalloc(&c->work, np * capacity * sizeof(float2))) != hipSuccess)
         return cleanup_fail("hipMalloc work", e);
     if ((e = hipMalloc(&c->supp, np * capacity)) != hipSuccess) return cleanup_fail("hipMalloc supp", e);
     if ((e = hipMalloc(&c->amp, np * sizeof(float))) != hipSuccess) return cleanup_fail("hipMalloc amp", e);
+    if ((e = hipMalloc(&c->amp_t, np * sizeof(float))) != hipSuccess) return cleanup_fail("hipMalloc amp_t", e);
     if ((e = hipMalloc(&c->tw, n * sizeof(float2))) != hipSuccess) return cleanup_fail("hipMalloc tw", e);
     if ((e = hipMalloc(&c->work2, np * capacity * sizeof(float2))) != hipSuccess)
         return cleanup_fail("hipMalloc work2", e);
@@ -633,6 +703,11 @@ int spr_create(spr_ctx** out, int n, int capacity, const float* amp, int device)
         return cleanup_fail("upload twiddles", e);
     if ((e = hipMemcpy(c->amp, amp, np * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
         return cleanup_fail("upload amplitude", e);
+    {
+        const std::vector<float> at = transpose_amp(amp, n);
+        if ((e = hipMemcpy(c->amp_t, at.data(), np * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
+            return cleanup_fail("upload amplitude", e);
+    }
     if ((e = hipMemset(c->supp, 0, np * capacity)) != hipSuccess) return cleanup_fail("clear supports", e);
     *out = c;
     return SPR_OK;
@@ -647,6 +722,7 @@ int spr_destroy(spr_ctx* c) {
     (void)hipFree(c->work);
     (void)hipFree(c->supp);
     (void)hipFree(c->amp);
+    (void)hipFree(c->amp_t);
     (void)hipFree(c->tw);
     (void)hipFree(c->work2);
     (void)hipFree(c->partial);
@@ -656,8 +732,12 @@ int spr_destroy(spr_ctx* c) {
     (void)hipFree(c->radix);
     (void)hipFree(c->hist);
     for (spr_ctx::Buf* b : {&c->spec, &c->xf, &c->xs, &c->supp2, &c->fieldbuf, &c->ampavg, &c->xidx, &c->shiftp,
-                            &c->xchild, &c->xmaps, &c->xphase, &c->adam_m, &c->adam_v})
+                            &c->xchild, &c->xmaps, &c->xphase, &c->adam_m, &c->adam_v, &c->gidx, &c->sstage})
         (void)hipFree(b->p);
+    if (c->lane2) (void)hipStreamSynchronize(c->lane2);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->lane2) (void)hipStreamDestroy(c->lane2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SPR_OK;
@@ -675,7 +755,10 @@ int spr_size(const spr_ctx* c, int* n, int* capacity) {
 int spr_set_pattern(spr_ctx* c, const float* amp) {
     if (!c || !amp) return fail(c, SPR_ERR_ARG, "NULL argument");
     SPR_HIP(c, hipSetDevice(c->device));
+    const std::vector<float> at = transpose_amp(amp, c->n);
     SPR_HIP(c, hipMemcpyAsync(c->amp, amp, (size_t)c->n * c->n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    SPR_HIP(c, hipMemcpyAsync(c->amp_t, at.data(), (size_t)c->n * c->n * sizeof(float), hipMemcpyHostToDevice,
+                              c->stream));
     SPR_HIP(c, hipStreamSynchronize(c->stream));
     return SPR_OK;
 }
@@ -721,6 +804,115 @@ int spr_get_population(spr_ctx* c, int P, float* rho, int32_t* support) {
         SPR_HIP(c, hipStreamSynchronize(c->stream));
         for (size_t i = 0; i < s.size(); ++i) support[i] = s[i];
     }
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    return SPR_OK;
+}
+
+int spr_set_active(spr_ctx* c, int P) {
+    if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
+    if (P < 0 || P > c->cap) return fail(c, SPR_ERR_ARG, "population size exceeds capacity");
+    c->P = P;
+    return SPR_OK;
+}
+
+int spr_gather(spr_ctx* c, int count, const int* src) {
+    if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
+    if (count < 0 || count > c->cap) return fail(c, SPR_ERR_ARG, "count exceeds capacity");
+    if (count > 0 && !src) return fail(c, SPR_ERR_ARG, "src is NULL");
+    std::vector<int> from, to;
+    for (int i = 0; i < count; ++i) {
+        if (src[i] < -1 || src[i] >= c->cap) return fail(c, SPR_ERR_ARG, "slot index out of range");
+        if (src[i] >= 0 && src[i] != i) {
+            from.push_back(src[i]);
+            to.push_back(i);
+        }
+    }
+    if (from.empty()) return SPR_OK;
+    const int m = (int)from.size();
+    const int np = c->n * c->n;
+    SPR_HIP(c, hipSetDevice(c->device));
+    SPR_TRY(ensure(c, c->gidx, 2 * m * sizeof(int)));
+    SPR_TRY(ensure(c, c->supp2, (size_t)m * np));
+    std::vector<int> idx(from);
+    idx.insert(idx.end(), to.begin(), to.end());
+    int* d_from = (int*)c->gidx.p;
+    int* d_to = d_from + m;
+    SPR_HIP(c, hipMemcpyAsync(d_from, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    // Every source is read into the staging fields before any slot is written.
+    const dim3 gf((np + 255) / 256, m), gs((np / 16 + 255) / 256, m);
+    hipLaunchKernelGGL(k_copy_fields, gf, dim3(256), 0, c->stream, (const float2*)c->rho, d_from, c->work,
+                       (const int*)nullptr, np);
+    hipLaunchKernelGGL(k_copy_supports, gs, dim3(256), 0, c->stream, (const uint4*)c->supp, d_from,
+                       (uint4*)c->supp2.p, (const int*)nullptr, np / 16);
+    hipLaunchKernelGGL(k_copy_fields, gf, dim3(256), 0, c->stream, (const float2*)c->work, (const int*)nullptr,
+                       c->rho, d_to, np);
+    hipLaunchKernelGGL(k_copy_supports, gs, dim3(256), 0, c->stream, (const uint4*)c->supp2.p, (const int*)nullptr,
+                       (uint4*)c->supp, d_to, np / 16);
+    SPR_HIP(c, hipGetLastError());
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    return SPR_OK;
+}
+
+int spr_put_individuals(spr_ctx* c, int count, const int* slots, const float* const* rho,
+                        const int32_t* const* support) {
+    if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
+    if (count < 0 || count > c->cap) return fail(c, SPR_ERR_ARG, "count exceeds capacity");
+    if (count == 0) return SPR_OK;
+    if (!slots) return fail(c, SPR_ERR_ARG, "slots is NULL");
+    for (int i = 0; i < count; ++i)
+        if (slots[i] < 0 || slots[i] >= c->cap) return fail(c, SPR_ERR_ARG, "slot index out of range");
+    const size_t np = (size_t)c->n * c->n;
+    SPR_HIP(c, hipSetDevice(c->device));
+    if (rho)
+        for (int i = 0; i < count; ++i)
+            if (rho[i])
+                SPR_HIP(c, hipMemcpyAsync(c->rho + (size_t)slots[i] * np, rho[i], np * sizeof(float2),
+                                          hipMemcpyHostToDevice, c->stream));
+    if (support) {
+        SPR_TRY(ensure(c, c->gidx, count * sizeof(int)));
+        SPR_TRY(ensure(c, c->sstage, (size_t)count * np * sizeof(int32_t)));
+        int32_t* stage = (int32_t*)c->sstage.p;
+        for (int i = 0; i < count; ++i) {
+            if (!support[i]) return fail(c, SPR_ERR_ARG, "support pointer is NULL");
+            SPR_HIP(c, hipMemcpyAsync(stage + (size_t)i * np, support[i], np * sizeof(int32_t),
+                                      hipMemcpyHostToDevice, c->stream));
+        }
+        SPR_HIP(c, hipMemcpyAsync(c->gidx.p, slots, count * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_supports_to_u8, dim3((unsigned)((np + 255) / 256), count), dim3(256), 0, c->stream,
+                           (const int*)stage, (const int*)c->gidx.p, c->supp, (int)np);
+        SPR_HIP(c, hipGetLastError());
+    }
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    return SPR_OK;
+}
+
+int spr_fetch_individuals(spr_ctx* c, int count, const int* slots, float* const* rho, int32_t* const* support) {
+    if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
+    if (count < 0 || count > c->cap) return fail(c, SPR_ERR_ARG, "count exceeds capacity");
+    if (count == 0) return SPR_OK;
+    if (!slots) return fail(c, SPR_ERR_ARG, "slots is NULL");
+    for (int i = 0; i < count; ++i)
+        if (slots[i] < 0 || slots[i] >= c->cap) return fail(c, SPR_ERR_ARG, "slot index out of range");
+    const size_t np = (size_t)c->n * c->n;
+    SPR_HIP(c, hipSetDevice(c->device));
+    if (support) {
+        SPR_TRY(ensure(c, c->gidx, count * sizeof(int)));
+        SPR_TRY(ensure(c, c->sstage, (size_t)count * np * sizeof(int32_t)));
+        int32_t* stage = (int32_t*)c->sstage.p;
+        SPR_HIP(c, hipMemcpyAsync(c->gidx.p, slots, count * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_supports_to_i32, dim3((unsigned)((np + 255) / 256), count), dim3(256), 0, c->stream,
+                           (const uint8_t*)c->supp, (const int*)c->gidx.p, stage, (int)np);
+        SPR_HIP(c, hipGetLastError());
+        for (int i = 0; i < count; ++i)
+            if (support[i])
+                SPR_HIP(c, hipMemcpyAsync(support[i], stage + (size_t)i * np, np * sizeof(int32_t),
+                                          hipMemcpyDeviceToHost, c->stream));
+    }
+    if (rho)
+        for (int i = 0; i < count; ++i)
+            if (rho[i])
+                SPR_HIP(c, hipMemcpyAsync(rho[i], c->rho + (size_t)slots[i] * np, np * sizeof(float2),
+                                          hipMemcpyDeviceToHost, c->stream));
     SPR_HIP(c, hipStreamSynchronize(c->stream));
     return SPR_OK;
 }
@@ -1225,10 +1417,13 @@ int spr_timing(spr_ctx* c, int enable) {
     if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
     SPR_HIP(c, hipStreamSynchronize(c->stream));
     SPR_TRY(harvest_timing(c));
+    if (enable < 0) return fail(c, SPR_ERR_ARG, "negative timing stride");
     c->timing = enable != 0;
+    c->timing_stride = enable > 0 ? enable : 1;
     for (int k = 0; k < SPR_TIMING_KINDS; ++k) {
         c->kind_ms[k] = 0;
         c->kind_count[k] = 0;
+        c->kind_seen[k] = 0;
     }
     return SPR_OK;
 }

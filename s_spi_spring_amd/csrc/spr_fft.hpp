@@ -161,7 +161,7 @@ __device__ __forceinline__ void stage_load(float2 (&v)[8], const Acc& acc, int t
 #pragma unroll
     for (int u = 0; u < NB; ++u)
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[u * R + r] = acc.ld(t + T * u + r * M);
+        for (int r = 0; r < R; ++r) v[u * R + r] = acc.template ldk<M>(t + T * u, r);
 }
 
 // Stage outputs of stage STAGE to LDS: y[(b/Ns)*Ns*R + b%Ns + q*Ns] = v[u*R + q].
@@ -180,33 +180,49 @@ __device__ __forceinline__ void stage_store(const float2 (&v)[8], const Acc& acc
     }
 }
 
+// Synchronisation between the LDS exchanges of a line transform: a workgroup
+// barrier, or, when every line lives inside one wavefront (WAVE), a wavefront
+// fence. LDS operations of one wavefront execute in program order, so the
+// fence only has to keep the compiler from moving LDS accesses across it.
+template <bool WAVE> __device__ __forceinline__ void line_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 // Full line transform. On entry v[r] = x[t + r*N/8]; on exit v[q] = X[t + q*N/8]
-// with X[k] = sum_j x[j] exp(SIGN*2*pi*i*j*k/N) (unnormalised). Every thread of
-// the workgroup must call it (it contains workgroup barriers).
-template <int N, int SIGN, class Acc>
+// with X[k] = sum_j x[j] exp(SIGN*2*pi*i*j*k/N) (unnormalised). With WAVE = false
+// every thread of the workgroup must call it (it contains workgroup barriers);
+// with WAVE = true the N/8 threads of each line must lie in one wavefront and
+// the line's LDS must be touched by that wavefront only.
+template <int N, int SIGN, bool WAVE = false, class Acc>
 __device__ __forceinline__ void fft_line(float2 (&v)[8], const Acc& acc, const float2* __restrict__ tw, int t) {
     constexpr int NSTAGE = RadixPlan<N>::ns;
     stage_compute<N, 0, SIGN>(v, tw, t);
     stage_store<N, 0>(v, acc, t);
-    __syncthreads();
+    line_sync<WAVE>();
     stage_load<N, 1>(v, acc, t);
     stage_compute<N, 1, SIGN>(v, tw, t);
     if constexpr (NSTAGE > 2) {
-        __syncthreads();
+        line_sync<WAVE>();
         stage_store<N, 1>(v, acc, t);
-        __syncthreads();
+        line_sync<WAVE>();
         stage_load<N, 2>(v, acc, t);
         stage_compute<N, 2, SIGN>(v, tw, t);
     }
     if constexpr (NSTAGE > 3) {
-        __syncthreads();
+        line_sync<WAVE>();
         stage_store<N, 2>(v, acc, t);
-        __syncthreads();
+        line_sync<WAVE>();
         stage_load<N, 3>(v, acc, t);
         stage_compute<N, 3, SIGN>(v, tw, t);
     }
     // Leave LDS free for the next user of this buffer.
-    __syncthreads();
+    line_sync<WAVE>();
 }
 
 }  // namespace spr

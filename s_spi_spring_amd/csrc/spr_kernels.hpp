@@ -34,11 +34,23 @@ template <int N> struct Geo {
     static constexpr int C = (8192 / N) < N ? (8192 / N) : N;  // columns per column-pass workgroup
     static constexpr int COL_THREADS = C * T;
     static constexpr int CS = C + 1;                      // padded LDS row length of a column tile
+    // Column-major LDS tile of the wavefront-owned column pass (T <= 64): column
+    // c, row i at c*CST + i + i/8. Within a column this is the row pass's padded
+    // layout; the column stride CST = RPW (mod 2*RPW) float2, RPW = 64/C rows
+    // per wavefront in the coalesced load layout, spreads a wavefront's C columns
+    // x RPW rows evenly over the LDS banks when the tile is transposed.
+    static constexpr int RPW = (C >= 64) ? 1 : 64 / C;
+    static constexpr int CST = ((N + N / 8) / (2 * RPW) + 1) * (2 * RPW) + RPW;
     // Two column workgroups per CU when their LDS fits (160 KB): waves per SIMD to target.
-    static constexpr int COL_LDS = (N * CS + TwTable<N>::SIZE) * 8 + 512;
+    static constexpr int COL_TILE = (T <= 64) ? C * CST : N * CS;   // float2 of the column tile
+    static constexpr int COL_LDS = (COL_TILE + TwTable<N>::SIZE) * 8 + 512;
 #ifndef SPR_COL_WG_PER_CU
 #define SPR_COL_WG_PER_CU 2
 #endif
+    // Row workgroups per CU allowed by LDS (4 waves each, one per SIMD): the
+    // register budget the row pass is compiled for.
+    static constexpr int ROW_LDS = (RW * RS + TwTable<N>::SIZE) * 8;
+    static constexpr int ROW_WAVES_PER_EU = (160 * 1024 / ROW_LDS) < 8 ? (160 * 1024 / ROW_LDS) : 8;
     static constexpr int COL_WAVES_PER_EU =
         (SPR_COL_WG_PER_CU * COL_LDS <= 160 * 1024) ? SPR_COL_WG_PER_CU * COL_THREADS / 256 : COL_THREADS / 256;
     static_assert(RW >= 1 && ROW_THREADS % T == 0, "row geometry");
@@ -49,12 +61,26 @@ struct RowAcc {
     float2* base;
     __device__ __forceinline__ float2 ld(int i) const { return base[i + (i >> 3)]; }
     __device__ __forceinline__ void st(int i, float2 v) const { base[i + (i >> 3)] = v; }
+    // Element i0 + k*STEP for STEP a multiple of 8: one base register plus
+    // constant offsets, since (i0 + k*STEP) >> 3 = (i0 >> 3) + k*STEP/8.
+    template <int STEP> __device__ __forceinline__ float2 ldk(int i0, int k) const {
+        static_assert(STEP % 8 == 0, "stride");
+        return base[i0 + (i0 >> 3) + k * (STEP + STEP / 8)];
+    }
+    template <int STEP> __device__ __forceinline__ void stk(int i0, int k, float2 v) const {
+        static_assert(STEP % 8 == 0, "stride");
+        base[i0 + (i0 >> 3) + k * (STEP + STEP / 8)] = v;
+    }
 };
 
 template <int CS> struct ColAcc {
     float2* base;
     __device__ __forceinline__ float2 ld(int i) const { return base[i * CS]; }
     __device__ __forceinline__ void st(int i, float2 v) const { base[i * CS] = v; }
+    template <int STEP> __device__ __forceinline__ float2 ldk(int i0, int k) const { return base[(i0 + k * STEP) * CS]; }
+    template <int STEP> __device__ __forceinline__ void stk(int i0, int k, float2 v) const {
+        base[(i0 + k * STEP) * CS] = v;
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -136,10 +162,17 @@ struct RowArgs {
     PhaseCut pc;
 };
 
+// Loads of the state rho_n and the support issued with the field, ahead of the
+// first transform: 0 none, 1 rho_n, 2 rho_n and the support.
+#ifndef SPR_ROW_PREFETCH
+#define SPR_ROW_PREFETCH 2
+#endif
+
 template <int N, int MODE>
 __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
     using G = Geo<N>;
     constexpr int T = G::T;
+    constexpr bool WV = T <= 64;   // every row inside one wavefront: no workgroup barriers
     __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[G::RW * G::RS];
     const int tid = threadIdx.x;
@@ -159,7 +192,7 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
     if constexpr (MODE == ROW_FWD || MODE == ROW_FWD_INPLACE) {
 #pragma unroll
         for (int r = 0; r < 8; ++r) v[r] = rho[off + r * T];
-        fft_line<N, +1>(v, acc, tw, t);
+        fft_line<N, +1, WV>(v, acc, tw, t);
         float2* dst = (MODE == ROW_FWD) ? work : rho;
 #pragma unroll
         for (int q = 0; q < 8; ++q) dst[off + q * T] = v[q];
@@ -167,29 +200,42 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
         const float2* src = (MODE == ROW_FINAL) ? work : rho;
 #pragma unroll
         for (int r = 0; r < 8; ++r) v[r] = src[off + r * T];
-        fft_line<N, -1>(v, acc, tw, t);
+        fft_line<N, -1, WV>(v, acc, tw, t);
 #pragma unroll
         for (int q = 0; q < 8; ++q) rho[off + q * T] = make_float2(v[q].x * a.inv_nn, v[q].y * a.inv_nn);
     } else {
+        // The state and the support may be loaded with the field, before the
+        // transform, so that their latency hides behind it.
+        float2 bn[8];
+        uint8_t sn[8];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = work[off + r * T];
-        fft_line<N, -1>(v, acc, tw, t);
+        for (int r = 0; r < 8; ++r) {
+            v[r] = work[off + r * T];
+            if constexpr (SPR_ROW_PREFETCH >= 2) sn[r] = supp[off + r * T];
+            if constexpr (MODE != ROW_ER && SPR_ROW_PREFETCH >= 1) bn[r] = rho[off + r * T];
+        }
+        fft_line<N, -1, WV>(v, acc, tw, t);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            if constexpr (SPR_ROW_PREFETCH < 2) sn[r] = supp[off + r * T];
+            if constexpr (MODE != ROW_ER && SPR_ROW_PREFETCH < 1) bn[r] = rho[off + r * T];
+        }
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int i = off + q * T;
             const float2 d = make_float2(v[q].x * a.inv_nn, v[q].y * a.inv_nn);  // rho'
-            const uint8_t s = supp[i];
+            const uint8_t s = sn[q];
             float2 o;
             if constexpr (MODE == ROW_ER) {
                 const float ts = (s != 0 && !phase_fails(d.x, d.y, a.pc)) ? 1.f : 0.f;
                 o = make_float2(d.x * ts, d.y * ts);
             } else if constexpr (MODE == ROW_HIO) {
-                const float2 b = rho[i];
+                const float2 b = bn[q];
                 const float ts = (s != 0 && !phase_fails(d.x, d.y, a.pc)) ? 1.f : 0.f;
                 o = make_float2((b.x - a.beta * d.x) * (1.f - ts) + d.x * ts,
                                 (b.y - a.beta * d.y) * (1.f - ts) + d.y * ts);
             } else {  // RAAR
-                const float2 b = rho[i];
+                const float2 b = bn[q];
                 const float qx = 2.f * d.x - b.x, qy = 2.f * d.y - b.y;
                 o = d;
                 if (s == 0 || phase_fails(qx, qy, a.pc)) {
@@ -200,7 +246,7 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
             if constexpr (MODE != ROW_ER) rho[i] = o;
             v[q] = o;
         }
-        fft_line<N, +1>(v, acc, tw, t);
+        fft_line<N, +1, WV>(v, acc, tw, t);
 #pragma unroll
         for (int q = 0; q < 8; ++q) work[off + q * T] = v[q];
     }
@@ -219,15 +265,24 @@ struct ColArgs {
     float2* f;          // field batch, transformed in place
     const float* amp;
     const float2* tw;
+    const float* amp_t; // the amplitude transposed, amp_t[x*N + y] = amp[y*N + x]
 };
 
+// Column pass over C adjacent columns of one field. Global memory is read and
+// written as rows of C consecutive values (coalesced). When a column's T = N/8
+// threads fit in one wavefront, the tile is transposed through LDS so that each
+// wavefront owns whole columns: its two transforms then exchange data with
+// wavefront fences instead of workgroup barriers, and the amplitude is read
+// from its transposed copy along the column (coalesced). Larger N keeps the
+// row-major thread layout with workgroup barriers.
 template <int N, int MODE>
 __global__ __launch_bounds__(Geo<N>::COL_THREADS) __attribute__((amdgpu_waves_per_eu(Geo<N>::COL_WAVES_PER_EU)))
 void k_col(ColArgs a) {
     using G = Geo<N>;
-    constexpr int T = G::T, C = G::C;
+    constexpr int T = G::T, C = G::C, CS = G::CS;
+    constexpr bool WV = T <= 64;
     __shared__ float2 tw[TwTable<N>::SIZE];
-    __shared__ float2 lds[N * G::CS];
+    __shared__ float2 lds[G::COL_TILE];
     const int tid = threadIdx.x;
     const int c = tid % C;
     const int t = tid / C;
@@ -237,19 +292,44 @@ void k_col(ColArgs a) {
     const int x0 = (blockIdx.x % TILES) * C;
     // Block-uniform bases with 32-bit per-thread offsets (SGPR base addressing).
     float2* __restrict__ f = a.f + (size_t)p * N * N + x0;
-    const float* __restrict__ amp = a.amp + x0;
     const int off = t * N + c;
-    const ColAcc<G::CS> acc{lds + c};
     float2 v[8];
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = f[off + r * T * N];
-    __syncthreads();
-    if constexpr (MODE == COL_PROJ || MODE == COL_FWD) fft_line<N, +1>(v, acc, tw, t);
-    if constexpr (MODE == COL_PROJ) {
+    if constexpr (WV) {
+        constexpr int CST = G::CST;
+        const int cc = tid / T;   // column owned by this thread's wavefront
+        const int tl = tid % T;
+        const RowAcc mine{lds + c * CST};
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = project_modulus(v[q], amp[off + q * T * N]);
+        for (int r = 0; r < 8; ++r) mine.stk<T>(t, r, v[r]);
+        __syncthreads();
+        const RowAcc acc{lds + cc * CST};
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = acc.ldk<T>(tl, r);
+        if constexpr (MODE == COL_PROJ || MODE == COL_FWD) fft_line<N, +1, true>(v, acc, tw, tl);
+        if constexpr (MODE == COL_PROJ) {
+            const float* __restrict__ at = a.amp_t + (x0 + cc) * N;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = project_modulus(v[q], at[tl + q * T]);
+        }
+        if constexpr (MODE == COL_PROJ || MODE == COL_INV) fft_line<N, -1, true>(v, acc, tw, tl);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc.stk<T>(tl, q, v[q]);
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = mine.ldk<T>(t, r);
+    } else {
+        const float* __restrict__ amp = a.amp + x0;
+        const ColAcc<CS> acc{lds + c};
+        __syncthreads();
+        if constexpr (MODE == COL_PROJ || MODE == COL_FWD) fft_line<N, +1>(v, acc, tw, t);
+        if constexpr (MODE == COL_PROJ) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = project_modulus(v[q], amp[off + q * T * N]);
+        }
+        if constexpr (MODE == COL_PROJ || MODE == COL_INV) fft_line<N, -1>(v, acc, tw, t);
     }
-    if constexpr (MODE == COL_PROJ || MODE == COL_INV) fft_line<N, -1>(v, acc, tw, t);
 #pragma unroll
     for (int q = 0; q < 8; ++q) f[off + q * T * N] = v[q];
 }

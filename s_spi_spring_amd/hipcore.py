@@ -34,6 +34,10 @@ SIGNATURES = {
     "spr_get_pattern": (_I, [_VP, _VP]),
     "spr_set_population": (_I, [_VP, _I, _VP, _VP]),
     "spr_get_population": (_I, [_VP, _I, _VP, _VP]),
+    "spr_set_active": (_I, [_VP, _I]),
+    "spr_gather": (_I, [_VP, _I, _VP]),
+    "spr_put_individuals": (_I, [_VP, _I, _VP, _VP, _VP]),
+    "spr_fetch_individuals": (_I, [_VP, _I, _VP, _VP, _VP]),
     "spr_ft": (_I, [_VP]),
     "spr_ift": (_I, [_VP]),
     "spr_iterate": (_I, [_VP, _I, _I, _F, _F]),
@@ -184,6 +188,37 @@ class Backend:
         self._check(self.lib.spr_get_population(self.h, P, _ptr(rho), _ptr(s)))
         return (rho, s) if support else rho
 
+    def set_active(self, P):
+        self._check(self.lib.spr_set_active(self.h, int(P)))
+        self.P = int(P)
+
+    def gather(self, src):
+        """Slot i <- slot src[i] (-1: leave slot i)."""
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        self._check(self.lib.spr_gather(self.h, len(src), _ptr(src)))
+
+    def put_individuals(self, slots, rho, support=None):
+        """Upload rho[k] (and support[k]) into slot slots[k]."""
+        slots = np.ascontiguousarray(slots, dtype=np.int32)
+        rho = [np.ascontiguousarray(r, dtype=np.complex64) for r in rho]
+        rp = (ctypes.c_void_p * len(rho))(*[r.ctypes.data for r in rho])
+        sp = None
+        if support is not None:
+            support = [np.ascontiguousarray(x, dtype=np.int32) for x in support]
+            sp = (ctypes.c_void_p * len(support))(*[x.ctypes.data for x in support])
+        self._check(self.lib.spr_put_individuals(self.h, len(slots), _ptr(slots), rp, sp))
+
+    def fetch_individuals(self, slots, support=True):
+        slots = np.ascontiguousarray(slots, dtype=np.int32)
+        rho = [np.empty((self.n, self.n), np.complex64) for _ in slots]
+        rp = (ctypes.c_void_p * len(rho))(*[r.ctypes.data for r in rho])
+        sup, sp = None, None
+        if support:
+            sup = [np.empty((self.n, self.n), np.int32) for _ in slots]
+            sp = (ctypes.c_void_p * len(sup))(*[x.ctypes.data for x in sup])
+        self._check(self.lib.spr_fetch_individuals(self.h, len(slots), _ptr(slots), rp, sp))
+        return (rho, sup) if support else rho
+
     def ft(self):
         self._check(self.lib.spr_ft(self.h))
 
@@ -262,8 +297,9 @@ class Backend:
         self._check(self.lib.spr_refine(self.h, int(steps), float(lr), int(kind), float(eps), _ptr(loss)))
         return loss
 
-    def timing(self, enable: bool):
-        self._check(self.lib.spr_timing(self.h, int(bool(enable))))
+    def timing(self, enable: bool, stride: int = 1):
+        """Bracket every `stride`-th launch of each kind with HIP events (0/False: off)."""
+        self._check(self.lib.spr_timing(self.h, int(stride) if enable else 0))
 
     def timing_read(self, kind):
         ms = ctypes.c_double()

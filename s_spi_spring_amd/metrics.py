@@ -3,10 +3,13 @@
 real_space_error(rho, truth): relative L2 distance min over the ambiguities of
 phase retrieval from a Fourier modulus — cyclic translation, global phase and
 the twin (complex conjugate point reflection) — of ||c rho' - truth|| / ||truth||
-with rho' the shifted (and possibly twinned) reconstruction and c the unit
-phase factor that minimises the distance (the alignment spring/result.py:748-790
-performs for saved reconstructions). The translation is the peak of the FFT
-cross-correlation of the complex fields; both twin choices are tried.
+with rho' the shifted (and possibly twinned) reconstruction (the alignment
+spring/result.py:748-790 performs for saved reconstructions). The translation
+is the peak of the FFT cross-correlation of the complex fields; both twin
+choices are tried. c is the least-squares complex factor <rho', truth> /
+<rho', rho'>: its phase is the global phase, and its modulus absorbs the
+overall scale, which a photon-count pattern fixes only up to the unknown
+total (scale=False keeps |c| = 1).
 """
 from __future__ import annotations
 
@@ -20,19 +23,22 @@ def _best_shift(a, b):
     return np.unravel_index(i, a.shape)
 
 
-def _aligned_error(a, truth):
+def _aligned_error(a, truth, scale):
     s = _best_shift(a, truth)
     a = np.roll(a, s, axis=(0, 1))
     ip = np.vdot(a, truth)
-    c = ip / abs(ip) if abs(ip) > 0 else 1.0
+    if abs(ip) == 0:
+        c = 1.0
+    else:
+        c = ip / np.vdot(a, a).real if scale else ip / abs(ip)
     return float(np.linalg.norm(c * a - truth) / np.linalg.norm(truth))
 
 
-def real_space_error(rho, truth):
+def real_space_error(rho, truth, scale=True):
     a = np.asarray(rho, dtype=np.complex128)
     t = np.asarray(truth, dtype=np.complex128)
     twin = np.conj(np.roll(a[::-1, ::-1], (1, 1), axis=(0, 1)))
-    return min(_aligned_error(a, t), _aligned_error(twin, t))
+    return min(_aligned_error(a, t, scale), _aligned_error(twin, t, scale))
 
 
 def fourier_error(rho, amp):

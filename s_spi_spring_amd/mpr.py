@@ -7,8 +7,10 @@ run the algorithm sequence on parents and children, and replace a parent by its
 child when the child's error is lower (mpr.py:338-347).
 
 Besides the reference's behaviour, run() records per generation the wall-clock
-time and the best Fourier error in `self.history`, and stops early when an
-optional `target_error` is reached (time-to-R_F measurements).
+time, the best Fourier error and the time spent in each phase (shrink-wrap,
+alignment and averaging, crossover, the two Sequences) in `self.history`, and
+stops early when an optional `target_error` is reached (time-to-R_F
+measurements).
 """
 from __future__ import annotations
 
@@ -124,7 +126,14 @@ class MPR:
         best = min(self.reconstructions, key=lambda r: r.get_error())
         self.history.append({"generation": self.generation, "time": time.perf_counter() - t0,
                              "best_error": float(best.get_error()),
-                             "best_fourier_error": float(best.get_fourier_error())})
+                             "best_fourier_error": float(best.get_fourier_error()),
+                             "phases": dict(self._phases)})
+        self._phases = {}
+
+    def _phase(self, name, t):
+        now = time.perf_counter()
+        self._phases[name] = self._phases.get(name, 0.0) + now - t
+        return now
 
     # --- main loop (mpr.py:205-383) --------------------------------------------
     def _sequence(self, generation, area, update_support_first=False):
@@ -140,36 +149,46 @@ class MPR:
         (optional) maps (generation, [best area, average area]) to the
         [(name, [params])] sequence of that generation instead of AlgManager."""
         self.schedule = schedule
+        self._phases = {}
         t0 = time.perf_counter()
         self.stats = Stats()
         self.SelectGPUs()
         self.InitSolver()
         self.InitIA()
+        t = time.perf_counter()
         self.InitPop()
+        t = self._phase("init", t)
         s = self.solver
         s.Evaluate(self.reconstructions)
         self.sort()
         best = self.reconstructions[self.sortedargs[0]]
         area = int(np.sum(best.get_support()))
         s.set_sequence(self._sequence(0, [area, area], self.settings_adv["sw_first"]))
+        t = self._phase("evaluate", t)
         self.reconstructions = s.Sequence(self.reconstructions)
+        t = self._phase("sequence", t)
         self.generation = 0
         self.replcount = 0
         self.sort()
         self.get_best()
         self.get_average()
+        t = self._phase("average", t)
         self.save_result()
+        self._phase("result", t)
         self._record(t0)
         gens = self.settings["global"]["generations"]
         while self.generation < gens:
             if target_error is not None and self.history[-1]["best_fourier_error"] < target_error:
                 break
             self.generation += 1
+            t = time.perf_counter()
             if not self.refine:
                 self.reconstructions = s.SW(self.reconstructions, *self.algman.get_support_params(self.generation))
+            t = self._phase("sw", t)
             self.sort()
             self.get_best()
             self.get_average()
+            t = self._phase("average", t)
             if not self.refine:
                 best_px = int(np.sum(self.bestrec.get_support()))
                 eval_px = int(np.sum(self.avgrec.get_support()))
@@ -182,9 +201,12 @@ class MPR:
             children = s.Crossover(self.reconstructions, self.bestrec, self.avgrec, ga["crossprob"],
                                    ga["crossweight"], self.settings_adv["crossexp"], self.settings_adv["crossft"],
                                    self.settings_adv["crosssym"], ga["crossaverage"], update_support=not self.refine)
+            t = self._phase("crossover", t)
             s.set_sequence(self._sequence(self.generation, suppix))
             self.reconstructions = s.Sequence(self.reconstructions)
+            t = self._phase("sequence", t)
             children = s.Sequence(children)
+            t = self._phase("sequence_children", t)
             repl = np.zeros(len(self.reconstructions))
             for i, child in enumerate(children):
                 j = i % len(self.reconstructions)
@@ -193,6 +215,7 @@ class MPR:
                     self.reconstructions[j] = child
             self.replcount = int(np.sum(repl))
             self.save_result()
+            self._phase("result", t)
             self._record(t0)
             if log:
                 h = self.history[-1]

@@ -6,7 +6,9 @@ plain_small   128^2 polygon: 10 blocks of 20 iterations (7 HIO(beta 0.9) blocks,
               after each block with sigma_k = 3 - 1.5 k/9.
 plain_512     512^2 ellipsoids: RAAR 400 (beta 0.87) + HIO 100 (beta 0.95) + ER 50 in
               25-iteration blocks, SW after each block (22 per generation,
-              sigma 3 -> 1.5, threshold 0.2).
+              sigma 3 -> 1.5, threshold SW_THRESHOLD_512 = 0.03, the reference's
+              default shrink-wrap threshold, spring/settings_manager.py:119;
+              at 0.2 the support of this object collapses within a generation).
 grad_512      the same blocks with a 20-step Adam refinement of the masked Poisson
               likelihood (lr 1e-2) after each RAAR block.
 
@@ -34,6 +36,9 @@ def plain_small(generation=0, area=None):
     return seq
 
 
+SW_THRESHOLD_512 = 0.03
+
+
 def _blocks_512(grad):
     blocks = []
     for _ in range(16):
@@ -42,19 +47,21 @@ def _blocks_512(grad):
     return blocks
 
 
-def plain_512(generation=0, area=None):
+def plain_512(generation=0, area=None, threshold=None):
     blocks = _blocks_512(False)
+    th = SW_THRESHOLD_512 if threshold is None else threshold
     seq = []
     for k, b in enumerate(blocks):
-        seq += b + [("SW", [_sigma(k, len(blocks)), 0.2])]
+        seq += b + [("SW", [_sigma(k, len(blocks)), th])]
     return seq
 
 
-def grad_512(generation=0, area=None):
+def grad_512(generation=0, area=None, threshold=None):
     blocks = _blocks_512(True)
+    th = SW_THRESHOLD_512 if threshold is None else threshold
     seq = []
     for k, b in enumerate(blocks):
-        seq += b + [("SW", [_sigma(k, len(blocks)), 0.2])]
+        seq += b + [("SW", [_sigma(k, len(blocks)), th])]
     return seq
 
 
@@ -73,11 +80,24 @@ def support_size(truth):
     return int(np.ceil(2 * np.sqrt(((x - n / 2) ** 2 + (y - n / 2) ** 2).max()) + 4))
 
 
-def time_to_rf(seed, form="plain", target=0.05, generations=10, popsize=64, n=512, photons=1e9):
+def noise_floor(I, truth):
+    """E_F of the object itself against its own noisy pattern, with the
+    least-squares scale: the lowest Fourier error a reconstruction can be
+    expected to reach."""
+    from . import synth
+    A = synth.corner_amplitude(I).astype(np.float64)
+    F = np.abs(synth.ft_plus(np.asarray(truth, np.complex128)))
+    k = A >= 0
+    s = (A[k] * F[k]).sum() / (F[k] ** 2).sum()
+    return float(np.sqrt(((A[k] - s * F[k]) ** 2).sum() / (A[k] ** 2).sum()))
+
+
+def time_to_rf(seed, form="plain", target=0.05, generations=10, popsize=64, n=512, photons=1e9, threshold=None):
     """One memetic run on the ellipsoid pattern of `seed`; returns the time to
     best E_F < target (None when not reached within `generations`), the
-    generation reached, and the final Fourier and real-space errors of the
-    best individual."""
+    generation reached, the final Fourier and real-space errors of the best
+    individual, the best E_F after every generation and the pattern's noise
+    floor."""
     from . import Pattern, Settings, metrics, synth
     from .mpr import MPR
     I, truth = synth.ellipsoids_pattern(seed=seed, n=n, photons=photons)
@@ -87,11 +107,15 @@ def time_to_rf(seed, form="plain", target=0.05, generations=10, popsize=64, n=51
     s.set_advanced("seed", 1000 + seed)
     m = MPR(Pattern.from_intensity(I), s)
     t0 = time.perf_counter()
-    res = m.run(target_error=target, log=False, schedule=SCHEDULES[form])
+    sched = SCHEDULES[form]
+    res = m.run(target_error=target, log=False,
+                schedule=lambda g, area: sched(g, area, threshold=threshold))
     wall = time.perf_counter() - t0
     h = m.history[-1]
     reached = h["best_fourier_error"] < target
     return {"seed": seed, "form": form, "time_to_target": h["time"] if reached else None,
             "wall": wall, "generation": h["generation"], "fourier_error": h["best_fourier_error"],
             "real_space_error": metrics.real_space_error(res["best_density"], truth),
-            "iterations": iterations(SCHEDULES[form]()) * (h["generation"] + 1)}
+            "history": [round(x["best_fourier_error"], 5) for x in m.history],
+            "noise_floor": noise_floor(I, truth),
+            "iterations": iterations(sched()) * (h["generation"] + 1)}

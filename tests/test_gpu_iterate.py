@@ -77,3 +77,33 @@ def test_twenty_iterations(hip, orc, alg, beta):
     assert e < 4 * max(e32, 1e-6)
     if alg == "ER":
         assert e < 1e-4
+
+
+def test_two_lane_iteration_is_bit_identical(hip, monkeypatch):
+    """The pipelined two-stream form (SPR_ITERATE_LANES=2, read at context
+    creation) runs the same per-individual arithmetic as the one-stream form."""
+    amp, rho, supp = case(256, 5, seed=11)
+    out = []
+    for lanes in ("1", "2"):
+        monkeypatch.setenv("SPR_ITERATE_LANES", lanes)
+        be = hip.Backend(amp, capacity=5)
+        be.set_population(rho, supp)
+        be.iterate("RAAR", 7, 0.87, 0.0)
+        be.iterate("HIO", 3, 0.9, 0.0)
+        out.append(be.get_population(support=False))
+        be.close()
+    assert np.array_equal(out[0], out[1])
+
+
+def test_sampled_timing_counts(hip):
+    amp, rho, supp = case(128, 2, seed=12)
+    be = hip.Backend(amp, capacity=2)
+    be.set_population(rho, supp)
+    for stride, want in ((1, 10), (4, 3)):
+        be.timing(True, stride=stride)
+        be.iterate("HIO", 10, 0.9, 0.0)
+        ms, cnt = be.timing_read(hip.KIND_ROW_UPDATE)
+        assert cnt == want and ms > 0
+    be.timing(False)
+    be.iterate("HIO", 2, 0.9, 0.0)
+    assert be.timing_read(hip.KIND_ROW_UPDATE)[1] == 0
