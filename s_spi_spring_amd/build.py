@@ -93,7 +93,7 @@ def build_core(verbose: bool = False) -> str:
     tmp = out + ".tmp"
     cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
            "-I", INCLUDE, "-I", CSRC, "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"],
-           os.path.join(CSRC, "solver.cpp"), os.path.join(CSRC, "spring_core.cpp"),
+           os.path.join(CSRC, "solver.cpp"), os.path.join(CSRC, "pattern_prep.cpp"), os.path.join(CSRC, "spring_core.cpp"),
            "-L", LIBDIR, "-lspring_hip", "-Wl,-rpath,$ORIGIN/lib", "-o", tmp]
     if verbose:
         print(" ".join(cmd))

@@ -363,7 +363,8 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_aux(AuxColArgs a) {
             }
         }
         // sign -1: (conj d) * r ; sign +1: d * r ; both weighted and masked
-#pragma unroll
+        // (one loop body: unrolling it would hold both products' operands at once)
+#pragma unroll 1
         for (int sgn = 0; sgn < 2; ++sgn) {
             const float sg = sgn == 0 ? -1.f : 1.f;
 #pragma unroll

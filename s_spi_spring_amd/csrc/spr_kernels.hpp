@@ -31,7 +31,14 @@ template <int N> struct Geo {
     static constexpr int ROW_THREADS = 256;
     static constexpr int RW = ROW_THREADS / T;            // rows per row-pass workgroup
     static constexpr int RS = N + N / 8;                  // padded LDS row length (float2)
-    static constexpr int C = (8192 / N) < N ? (8192 / N) : N;  // columns per column-pass workgroup
+#ifndef SPR_COL_COLUMNS
+#define SPR_COL_COLUMNS 8192   // columns x rows of a column-pass tile
+#endif
+#ifndef SPR_COL_WAVE
+#define SPR_COL_WAVE 0         // 1: wavefront-owned columns when N/8 <= 64
+#endif
+    static constexpr int C = (SPR_COL_COLUMNS / N) < N ? (SPR_COL_COLUMNS / N) : N;  // columns per column-pass workgroup
+    static constexpr bool COL_WAVE = SPR_COL_WAVE && T <= 64;
     static constexpr int COL_THREADS = C * T;
     static constexpr int CS = C + 1;                      // padded LDS row length of a column tile
     // Column-major LDS tile of the wavefront-owned column pass (T <= 64): column
@@ -42,17 +49,22 @@ template <int N> struct Geo {
     static constexpr int RPW = (C >= 64) ? 1 : 64 / C;
     static constexpr int CST = ((N + N / 8) / (2 * RPW) + 1) * (2 * RPW) + RPW;
     // Two column workgroups per CU when their LDS fits (160 KB): waves per SIMD to target.
-    static constexpr int COL_TILE = (T <= 64) ? C * CST : N * CS;   // float2 of the column tile
+    static constexpr int COL_TILE = COL_WAVE ? C * CST : N * CS;   // float2 of the column tile
     static constexpr int COL_LDS = (COL_TILE + TwTable<N>::SIZE) * 8 + 512;
 #ifndef SPR_COL_WG_PER_CU
-#define SPR_COL_WG_PER_CU 2
+#define SPR_COL_WG_PER_CU 8
 #endif
     // Row workgroups per CU allowed by LDS (4 waves each, one per SIMD): the
     // register budget the row pass is compiled for.
     static constexpr int ROW_LDS = (RW * RS + TwTable<N>::SIZE) * 8;
     static constexpr int ROW_WAVES_PER_EU = (160 * 1024 / ROW_LDS) < 8 ? (160 * 1024 / ROW_LDS) : 8;
-    static constexpr int COL_WAVES_PER_EU =
-        (SPR_COL_WG_PER_CU * COL_LDS <= 160 * 1024) ? SPR_COL_WG_PER_CU * COL_THREADS / 256 : COL_THREADS / 256;
+    // Workgroups per CU: as many as LDS and the 32 wave slots of a CU allow,
+    // at most SPR_COL_WG_PER_CU; the register budget follows from it.
+    static constexpr int COL_WG_LDS = (160 * 1024) / COL_LDS;
+    static constexpr int COL_WG_SLOTS = 32 / (COL_THREADS / 64);
+    static constexpr int COL_WG = COL_WG_LDS < COL_WG_SLOTS ? (COL_WG_LDS < SPR_COL_WG_PER_CU ? COL_WG_LDS : SPR_COL_WG_PER_CU)
+                                                            : (COL_WG_SLOTS < SPR_COL_WG_PER_CU ? COL_WG_SLOTS : SPR_COL_WG_PER_CU);
+    static constexpr int COL_WAVES_PER_EU = (COL_WG > 0 ? COL_WG : 1) * COL_THREADS / 256;
     static_assert(RW >= 1 && ROW_THREADS % T == 0, "row geometry");
     static_assert(COL_THREADS <= 1024, "column geometry");
 };
@@ -163,7 +175,9 @@ struct RowArgs {
 };
 
 // Loads of the state rho_n and the support issued with the field, ahead of the
-// first transform: 0 none, 1 rho_n, 2 rho_n and the support.
+// first transform: 0 none, 1 rho_n, 2 rho_n and the support. At 512^2 the RAAR
+// row pass measured 85 us per 64-field launch with 2 (6 waves/SIMD) against
+// 93 us with 0 (7 waves/SIMD).
 #ifndef SPR_ROW_PREFETCH
 #define SPR_ROW_PREFETCH 2
 #endif
@@ -216,26 +230,21 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
         }
         fft_line<N, -1, WV>(v, acc, tw, t);
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
-            if constexpr (SPR_ROW_PREFETCH < 2) sn[r] = supp[off + r * T];
-            if constexpr (MODE != ROW_ER && SPR_ROW_PREFETCH < 1) bn[r] = rho[off + r * T];
-        }
-#pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int i = off + q * T;
             const float2 d = make_float2(v[q].x * a.inv_nn, v[q].y * a.inv_nn);  // rho'
-            const uint8_t s = sn[q];
+            const uint8_t s = SPR_ROW_PREFETCH >= 2 ? sn[q] : supp[i];
             float2 o;
             if constexpr (MODE == ROW_ER) {
                 const float ts = (s != 0 && !phase_fails(d.x, d.y, a.pc)) ? 1.f : 0.f;
                 o = make_float2(d.x * ts, d.y * ts);
             } else if constexpr (MODE == ROW_HIO) {
-                const float2 b = bn[q];
+                const float2 b = SPR_ROW_PREFETCH >= 1 ? bn[q] : rho[i];
                 const float ts = (s != 0 && !phase_fails(d.x, d.y, a.pc)) ? 1.f : 0.f;
                 o = make_float2((b.x - a.beta * d.x) * (1.f - ts) + d.x * ts,
                                 (b.y - a.beta * d.y) * (1.f - ts) + d.y * ts);
             } else {  // RAAR
-                const float2 b = bn[q];
+                const float2 b = SPR_ROW_PREFETCH >= 1 ? bn[q] : rho[i];
                 const float qx = 2.f * d.x - b.x, qy = 2.f * d.y - b.y;
                 o = d;
                 if (s == 0 || phase_fails(qx, qy, a.pc)) {
@@ -269,18 +278,18 @@ struct ColArgs {
 };
 
 // Column pass over C adjacent columns of one field. Global memory is read and
-// written as rows of C consecutive values (coalesced). When a column's T = N/8
-// threads fit in one wavefront, the tile is transposed through LDS so that each
-// wavefront owns whole columns: its two transforms then exchange data with
-// wavefront fences instead of workgroup barriers, and the amplitude is read
-// from its transposed copy along the column (coalesced). Larger N keeps the
-// row-major thread layout with workgroup barriers.
+// written as rows of C consecutive values (coalesced); the C columns' transforms
+// exchange data through a padded LDS tile with workgroup barriers. With
+// SPR_COL_WAVE and N/8 <= 64 the tile is instead transposed through LDS so that
+// each wavefront owns whole columns and its transforms synchronise with
+// wavefront fences, reading the amplitude from its transposed copy; at 512^2
+// this form measured 71 us per 64-field launch against 66 us for the default.
 template <int N, int MODE>
 __global__ __launch_bounds__(Geo<N>::COL_THREADS) __attribute__((amdgpu_waves_per_eu(Geo<N>::COL_WAVES_PER_EU)))
 void k_col(ColArgs a) {
     using G = Geo<N>;
     constexpr int T = G::T, C = G::C, CS = G::CS;
-    constexpr bool WV = T <= 64;
+    constexpr bool WV = G::COL_WAVE;
     __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[G::COL_TILE];
     const int tid = threadIdx.x;

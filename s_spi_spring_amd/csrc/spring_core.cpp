@@ -1,6 +1,7 @@
 // pybind11 module `spring_core` with the Python surface of the reference's
-// binding (src/wrapper.cpp:24-312, 459-514): submodules `phase_retrieval`
-// (Reconstruction, Solver) and `GPUtils`. Argument names, defaults and the
+// binding (src/wrapper.cpp:24-420, 459-514): submodules `phase_retrieval`
+// (Reconstruction, Solver), `pattern` (detector-image preprocessing) and
+// `GPUtils`. Argument names, defaults and the
 // single-or-list convention of every method are the reference's.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -8,6 +9,7 @@
 
 #include <variant>
 
+#include "pattern_prep.h"
 #include "solver.h"
 
 namespace py = pybind11;
@@ -183,6 +185,68 @@ PYBIND11_MODULE(spring_core, sp) {
              },
              py::arg("reconstruction"), py::arg("ft") = true, py::arg("fraction") = 1.);
 
+    // Detector-image preprocessing (src/wrapper.cpp:318-420). Images are 2-D
+    // float64 arrays [rows][columns]; sizes and centres are (x, y).
+    py::module pm = sp.def_submodule("pattern");
+    using Dbl = py::array_t<double, py::array::c_style | py::array::forcecast>;
+    using Sz2 = std::array<std::size_t, 2>;
+    using D2 = std::array<double, 2>;
+    auto image = [](const std::vector<double>& v, Sz2 size) {
+        py::array_t<double> a({size[1], size[0]});
+        std::memcpy(a.mutable_data(), v.data(), v.size() * sizeof(double));
+        return a;
+    };
+    auto dims_of = [](const py::buffer_info& b, const char* what) {
+        if (b.ndim != 2) throw std::invalid_argument(std::string(what) + " must be 2-D");
+        return Sz2{(std::size_t)b.shape[1], (std::size_t)b.shape[0]};
+    };
+    auto check_mask = [](const py::buffer_info& m, Sz2 d) {
+        if (m.ndim != 2 || (std::size_t)m.shape[1] != d[0] || (std::size_t)m.shape[0] != d[1])
+            throw std::invalid_argument("mask must have the shape of data");
+    };
+    pm.def("downscale", [=](Dbl data, Sz2 newsize) {
+        auto b = data.request();
+        const Sz2 d = dims_of(b, "data");
+        return image(prep::downscale((const double*)b.ptr, d[0], d[1], newsize), newsize);
+    }, "Downscale the pattern to newsize (x, y); negative pixels are ignored.", py::arg("data"), py::arg("newsize"));
+    pm.def("downscale_and_correct", [=](Dbl data, Dbl mask, D2 center, double R, double dx, Sz2 newsize) {
+        auto b = data.request();
+        auto m = mask.request();
+        const Sz2 d = dims_of(b, "data");
+        check_mask(m, d);
+        return image(prep::downscale_and_correct((const double*)b.ptr, (const double*)m.ptr, d[0], d[1], center, R,
+                                                 dx, newsize), newsize);
+    }, "Ewald-sphere correction and downscaling of the pattern to newsize.", py::arg("data"), py::arg("mask"),
+       py::arg("center"), py::arg("distance"), py::arg("pixel_size"), py::arg("newsize"));
+    pm.def("cut_downscale_and_correct",
+           [=](Dbl data, Dbl mask, D2 center, double R, double dx, Sz2 cutsize, Sz2 newsize) {
+        auto b = data.request();
+        auto m = mask.request();
+        const Sz2 d = dims_of(b, "data");
+        check_mask(m, d);
+        auto r = prep::cut_downscale_and_correct((const double*)b.ptr, (const double*)m.ptr, d[0], d[1], center, R, dx,
+                                                 cutsize, newsize);
+        return std::make_tuple(image(std::get<0>(r), newsize), image(std::get<1>(r), newsize));
+    }, "Ewald-sphere correction, crop of cutsize around the centre and downscaling to newsize; returns "
+       "(pattern, mask).", py::arg("data"), py::arg("mask"), py::arg("center"), py::arg("distance"),
+       py::arg("pixel_size"), py::arg("cutsize"), py::arg("newsize"));
+    pm.def("cut_and_downscale", [=](Dbl data, Dbl mask, D2 center, Sz2 cutsize, Sz2 newsize) {
+        auto b = data.request();
+        auto m = mask.request();
+        const Sz2 d = dims_of(b, "data");
+        check_mask(m, d);
+        auto r = prep::cut_and_downscale((const double*)b.ptr, (const double*)m.ptr, d[0], d[1], center, cutsize,
+                                         newsize);
+        return std::make_tuple(image(std::get<0>(r), newsize), image(std::get<1>(r), newsize));
+    }, "Crop of cutsize around the centre and downscaling to newsize; returns (pattern, mask).", py::arg("data"),
+       py::arg("mask"), py::arg("center"), py::arg("cutsize"), py::arg("newsize"));
+    pm.def("get_radial_profile", [=](Dbl data, double dtheta, D2 center, std::vector<double> angles, D2 phi) {
+        auto b = data.request();
+        const Sz2 d = dims_of(b, "data");
+        return prep::radial_profile((const double*)b.ptr, d[0], d[1], dtheta, center, angles, phi);
+    }, "Radial profile at the given (increasing) scattering angles within the directions phi (degrees).",
+       py::arg("data"), py::arg("dtheta"), py::arg("center"), py::arg("angles"), py::arg("phi"));
+
     // Device enumeration (src/gpu/gpu_utils.cu:13-111 through src/wrapper.cpp:459-487).
     // There is no load query: get_usage returns -1 per device, as the reference
     // does when built without NVML.
@@ -222,4 +286,5 @@ PYBIND11_MODULE(spring_core, sp) {
     const std::string base = py::str(sp.attr("__name__"));
     modules[py::str(base + ".phase_retrieval")] = pr;
     modules[py::str(base + ".GPUtils")] = gu;
+    modules[py::str(base + ".pattern")] = pm;
 }
