@@ -363,11 +363,11 @@ RecVec Solver::Reshift(RecVec rec, RecPtr ref) {
     place(rec);
     const FLOAT ref_phase = field_phase(ref->density);
     std::vector<float> direct(P), flip(P);
-    for (int i = 0; i < P; ++i) {
+    parallel_for(P, [&](int i) {
         const FLOAT ph = field_phase(rec[i]->density);
         direct[i] = ref_phase - ph;
         flip[i] = ref_phase + ph;
-    }
+    });
     check(spr_align(ctx, reinterpret_cast<const float*>(ref->density.data()), shiftmin, shiftmax, direct.data(),
                     flip.data(), nullptr));
     pull(rec, true, true);
@@ -427,7 +427,7 @@ RecVec Solver::Crossover(RecVec rec, RecPtr best, RecPtr avg, FLOAT crossbalance
     const std::vector<FLOAT> avg_mod = GetAverageModule(rec, true, 0.66f);
 
     std::vector<FLOAT> phase(P);
-    for (int i = 0; i < P; ++i) phase[i] = field_phase(rec[i]->density);
+    parallel_for(P, [&](int i) { phase[i] = field_phase(rec[i]->density); });
     const FLOAT avg_phase = field_phase(avg->density);
     const int n = (int)dims[0];
     std::vector<spr_cross_desc> desc(P);
@@ -465,21 +465,22 @@ RecVec Solver::Crossover(RecVec rec, RecPtr best, RecPtr avg, FLOAT crossbalance
         }
     }
     place(rec);
-    std::vector<COMPLEX> child((std::size_t)P * size);
-    std::vector<SUPPORTTYPE> child_s((std::size_t)P * size);
+    // Child buffers are written completely by spr_crossover: no initialisation.
+    std::unique_ptr<float[]> child(new float[(std::size_t)P * size * 2]);
+    std::unique_ptr<SUPPORTTYPE[]> child_s(new SUPPORTTYPE[(std::size_t)P * size]);
     check(spr_crossover(ctx, P, desc.data(), maps.data(), (int)maps.size(),
                         reinterpret_cast<const float*>(avg->density.data()), avg_mod.data(), crossdiff, use_average,
-                        shiftmin, shiftmax, reality, update_support ? 1 : 0, reinterpret_cast<float*>(child.data()),
-                        child_s.data()));
-    for (int i = 0; i < P; ++i) {
-        RecPtr son = std::make_shared<Reconstruction>(*rec[i]);
-        std::copy(child.begin() + (std::size_t)i * size, child.begin() + (std::size_t)(i + 1) * size,
-                  son->density.begin());
-        if (update_support)
-            std::copy(child_s.begin() + (std::size_t)i * size, child_s.begin() + (std::size_t)(i + 1) * size,
-                      son->support.begin());
+                        shiftmin, shiftmax, reality, update_support ? 1 : 0, child.get(), child_s.get()));
+    // Each child starts as the reference's copy of rec[i] (field values, error and
+    // error_real; src/reconstruction.h:35-45) with the crossed density (and support).
+    parallel_for(P, [&](int i) {
+        const COMPLEX* d = reinterpret_cast<const COMPLEX*>(child.get()) + (std::size_t)i * size;
+        const SUPPORTTYPE* sp = update_support ? child_s.get() + (std::size_t)i * size : rec[i]->support.data();
+        RecPtr son = std::make_shared<Reconstruction>(d, sp, dims);
+        son->error = rec[i]->error;
+        son->error_real = rec[i]->error_real;
         out[i] = son;
-    }
+    });
     return out;
 }
 

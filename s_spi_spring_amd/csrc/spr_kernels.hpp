@@ -37,6 +37,9 @@ template <int N> struct Geo {
 #ifndef SPR_COL_WAVE
 #define SPR_COL_WAVE 0         // 1: wavefront-owned columns when N/8 <= 64
 #endif
+#ifndef SPR_COL_AMP_PREFETCH
+#define SPR_COL_AMP_PREFETCH 1 // load the amplitude with the field, before the forward transform
+#endif
     static constexpr int C = (SPR_COL_COLUMNS / N) < N ? (SPR_COL_COLUMNS / N) : N;  // columns per column-pass workgroup
     static constexpr bool COL_WAVE = SPR_COL_WAVE && T <= 64;
     static constexpr int COL_THREADS = C * T;
@@ -68,6 +71,32 @@ template <int N> struct Geo {
     static_assert(RW >= 1 && ROW_THREADS % T == 0, "row geometry");
     static_assert(COL_THREADS <= 1024, "column geometry");
 };
+
+// Loads and stores of the streamed fields (each value is read and written once
+// per pass). With SPR_NT they carry the non-temporal hint.
+#ifndef SPR_NT
+#define SPR_NT 0
+#endif
+__device__ __forceinline__ float2 ld_stream(const float2* p) {
+    if constexpr (SPR_NT) {
+        using v2 = float __attribute__((ext_vector_type(2)));
+        const v2 x = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+        return make_float2(x.x, x.y);
+    } else {
+        return *p;
+    }
+}
+__device__ __forceinline__ void st_stream(float2* p, float2 v) {
+    if constexpr (SPR_NT) {
+        using v2 = float __attribute__((ext_vector_type(2)));
+        v2 x;
+        x.x = v.x;
+        x.y = v.y;
+        __builtin_nontemporal_store(x, reinterpret_cast<v2*>(p));
+    } else {
+        *p = v;
+    }
+}
 
 struct RowAcc {
     float2* base;
@@ -224,9 +253,9 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
         uint8_t sn[8];
 #pragma unroll
         for (int r = 0; r < 8; ++r) {
-            v[r] = work[off + r * T];
+            v[r] = ld_stream(work + off + r * T);
             if constexpr (SPR_ROW_PREFETCH >= 2) sn[r] = supp[off + r * T];
-            if constexpr (MODE != ROW_ER && SPR_ROW_PREFETCH >= 1) bn[r] = rho[off + r * T];
+            if constexpr (MODE != ROW_ER && SPR_ROW_PREFETCH >= 1) bn[r] = ld_stream(rho + off + r * T);
         }
         fft_line<N, -1, WV>(v, acc, tw, t);
 #pragma unroll
@@ -252,12 +281,12 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
                     o = make_float2(a.beta * b.x + c * d.x, a.beta * b.y + c * d.y);
                 }
             }
-            if constexpr (MODE != ROW_ER) rho[i] = o;
+            if constexpr (MODE != ROW_ER) st_stream(rho + i, o);
             v[q] = o;
         }
         fft_line<N, +1, WV>(v, acc, tw, t);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) work[off + q * T] = v[q];
+        for (int q = 0; q < 8; ++q) st_stream(work + off + q * T, v[q]);
     }
 }
 
@@ -304,7 +333,7 @@ void k_col(ColArgs a) {
     const int off = t * N + c;
     float2 v[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = f[off + r * T * N];
+    for (int r = 0; r < 8; ++r) v[r] = ld_stream(f + off + r * T * N);
     if constexpr (WV) {
         constexpr int CST = G::CST;
         const int cc = tid / T;   // column owned by this thread's wavefront
@@ -331,16 +360,22 @@ void k_col(ColArgs a) {
     } else {
         const float* __restrict__ amp = a.amp + x0;
         const ColAcc<CS> acc{lds + c};
+        float am[8];
+        if constexpr (MODE == COL_PROJ && SPR_COL_AMP_PREFETCH) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) am[q] = amp[off + q * T * N];
+        }
         __syncthreads();
         if constexpr (MODE == COL_PROJ || MODE == COL_FWD) fft_line<N, +1>(v, acc, tw, t);
         if constexpr (MODE == COL_PROJ) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = project_modulus(v[q], amp[off + q * T * N]);
+            for (int q = 0; q < 8; ++q)
+                v[q] = project_modulus(v[q], SPR_COL_AMP_PREFETCH ? am[q] : amp[off + q * T * N]);
         }
         if constexpr (MODE == COL_PROJ || MODE == COL_INV) fft_line<N, -1>(v, acc, tw, t);
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) f[off + q * T * N] = v[q];
+    for (int q = 0; q < 8; ++q) st_stream(f + off + q * T * N, v[q]);
 }
 
 }  // namespace spr
