@@ -155,7 +155,7 @@ enum spr_op_kind { SPR_OP_ER = 0, SPR_OP_HIO = 1, SPR_OP_RAAR = 2, SPR_OP_SW = 3
 typedef struct spr_op {
     int kind;          /* spr_op_kind */
     int iterations;    /* ER/HIO/RAAR: iterations; GRAD: Adam steps */
-    float beta;        /* HIO/RAAR: beta; GRAD: learning rate */
+    float beta;        /* HIO/RAAR: beta; GRAD: step relative to the RMS density (spr_refine_support) */
     float sigma;       /* SW/SWA */
     float threshold;   /* SW: threshold; GRAD: loss eps */
     int pixels;        /* SWA: support pixels; GRAD: loss kind (spr_loss_grad) */
@@ -225,6 +225,13 @@ int spr_loss_grad(spr_ctx* ctx, int P, const float* rho, float* grad, double* lo
  * of the loss above on every resident density. loss (optional, host, P
  * doubles): the loss after the last step. */
 int spr_refine(spr_ctx* ctx, int steps, float lr, int kind, float eps, double* loss);
+
+/* Support-projected refinement (the GRAD operation of spr_run): as spr_refine,
+ * but only pixels inside each individual's support move (the gradient is
+ * masked by the support), and the step size of individual p is
+ * lr_rel * sqrt(sum_S |rho_p|^2 / |S_p|), the RMS density on its support when
+ * the call starts. No counterpart in the reference (new operator). */
+int spr_refine_support(spr_ctx* ctx, int steps, float lr_rel, int kind, float eps, double* loss);
 
 /* Device addresses of the resident population: rho (P*n*n float2) and the
  * uint8 supports; for callers that fill or read HBM directly. The amplitude

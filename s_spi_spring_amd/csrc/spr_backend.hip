@@ -1036,7 +1036,9 @@ int spr_run(spr_ctx* c, const spr_op* ops, int nops, int nrep, float reality) {
                 case SPR_OP_RAAR: SPR_TRY(queue_iterate(c, SPR_RAAR, o.iterations, o.beta, reality)); break;
                 case SPR_OP_SW: SPR_TRY(queue_sw(c, o.sigma, o.threshold)); break;
                 case SPR_OP_SWA: SPR_TRY(queue_swa(c, o.sigma, o.pixels)); break;
-                case SPR_OP_GRAD: SPR_TRY(spr_refine(c, o.iterations, o.beta, o.pixels, o.threshold, nullptr)); break;
+                case SPR_OP_GRAD:
+                    SPR_TRY(spr_refine_support(c, o.iterations, o.beta, o.pixels, o.threshold, nullptr));
+                    break;
             }
         }
     }
@@ -1376,6 +1378,46 @@ int spr_refine(spr_ctx* c, int steps, float lr, int kind, float eps, double* los
         hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->rho,
                            (const float2*)c->work2, (float2*)c->adam_m.p, (float2*)c->adam_v.p, n, lr, b1, b2, aeps,
                            1.f - p1, 1.f - p2);
+        SPR_HIP(c, hipGetLastError());
+    }
+    if (loss) {
+        SPR_TRY(queue_loss_grad(c, P, c->rho, c->work2, kind, eps));
+        std::vector<double> sm((size_t)P * SUMS_PER_ITEM);
+        SPR_HIP(c, hipMemcpyAsync(sm.data(), c->sums, sm.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        SPR_HIP(c, hipStreamSynchronize(c->stream));
+        for (int p = 0; p < P; ++p) loss[p] = sm[(size_t)p * SUMS_PER_ITEM + 7];
+    }
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    return harvest_timing(c);
+}
+
+int spr_refine_support(spr_ctx* c, int steps, float lr_rel, int kind, float eps, double* loss) {
+    if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
+    if (steps < 0) return fail(c, SPR_ERR_ARG, "negative step count");
+    if (kind != LOSS_POISSON && kind != LOSS_AMPLITUDE) return fail(c, SPR_ERR_ARG, "unknown loss kind");
+    const int P = c->P;
+    if (P <= 0 || steps == 0) return SPR_OK;
+    const int np = c->n * c->n;
+    const size_t n = (size_t)np * P;
+    SPR_HIP(c, hipSetDevice(c->device));
+    SPR_TRY(ensure(c, c->adam_m, n * sizeof(float2)));
+    SPR_TRY(ensure(c, c->adam_v, n * sizeof(float2)));
+    SPR_TRY(ensure(c, c->xphase, (size_t)P * sizeof(float)));
+    SPR_TRY(fill_u32(c, (unsigned*)c->adam_m.p, 0u, (int)(2 * n)));
+    SPR_TRY(fill_u32(c, (unsigned*)c->adam_v.p, 0u, (int)(2 * n)));
+    float* lrp = (float*)c->xphase.p;
+    hipLaunchKernelGGL(k_support_scale, dim3(P), dim3(256), 0, c->stream, (const float2*)c->rho,
+                       (const uint8_t*)c->supp, np, lr_rel, lrp);
+    SPR_HIP(c, hipGetLastError());
+    const float b1 = 0.9f, b2 = 0.999f, aeps = 1e-8f;
+    float p1 = 1.f, p2 = 1.f;
+    for (int s = 1; s <= steps; ++s) {
+        SPR_TRY(queue_loss_grad(c, P, c->rho, c->work2, kind, eps));
+        p1 *= b1;
+        p2 *= b2;
+        hipLaunchKernelGGL(k_adam_support, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->rho,
+                           (const float2*)c->work2, (float2*)c->adam_m.p, (float2*)c->adam_v.p,
+                           (const uint8_t*)c->supp, (const float*)lrp, np, n, b1, b2, aeps, 1.f - p1, 1.f - p2);
         SPR_HIP(c, hipGetLastError());
     }
     if (loss) {

@@ -91,4 +91,52 @@ __global__ __launch_bounds__(256) void k_adam(float2* x, const float2* g, float2
     x[i] = xi;
 }
 
+// Per-individual step size of the support-projected refinement:
+// lr_p = lr_rel * sqrt(sum_{S} |rho|^2 / |S|) (0 for an empty support), summed
+// in double in a fixed order (one workgroup per individual).
+__global__ __launch_bounds__(256) void k_support_scale(const float2* rho, const uint8_t* supp, int np, float lr_rel,
+                                                       float* lrp) {
+    const int p = blockIdx.x;
+    const float2* r = rho + (size_t)p * np;
+    const uint8_t* s = supp + (size_t)p * np;
+    double e = 0, cnt = 0;
+    for (int i = threadIdx.x; i < np; i += 256)
+        if (s[i]) {
+            e += (double)r[i].x * r[i].x + (double)r[i].y * r[i].y;
+            cnt += 1;
+        }
+    __shared__ double se[256], sc[256];
+    se[threadIdx.x] = e;
+    sc[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            se[threadIdx.x] += se[threadIdx.x + w];
+            sc[threadIdx.x] += sc[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lrp[p] = sc[0] > 0 ? (float)(lr_rel * sqrt(se[0] / sc[0])) : 0.f;
+}
+
+// Adam step restricted to the support: pixels outside it keep their value (and
+// zero moments), as torch.optim.Adam does for a gradient masked by the support;
+// the step size of individual p is lrp[p].
+__global__ __launch_bounds__(256) void k_adam_support(float2* x, const float2* g, float2* m, float2* v,
+                                                      const uint8_t* supp, const float* lrp, int np, size_t n,
+                                                      float b1, float b2, float eps, float bc1, float bc2) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n || !supp[i]) return;
+    const float lr = lrp[i / np];
+    const float2 gi = g[i];
+    float2 mi = m[i], vi = v[i], xi = x[i];
+    mi = make_float2(b1 * mi.x + (1.f - b1) * gi.x, b1 * mi.y + (1.f - b1) * gi.y);
+    vi = make_float2(b2 * vi.x + (1.f - b2) * gi.x * gi.x, b2 * vi.y + (1.f - b2) * gi.y * gi.y);
+    xi.x -= lr * (mi.x / bc1) / (sqrtf(vi.x / bc2) + eps);
+    xi.y -= lr * (mi.y / bc1) / (sqrtf(vi.y / bc2) + eps);
+    m[i] = mi;
+    v[i] = vi;
+    x[i] = xi;
+}
+
 }  // namespace spr

@@ -60,3 +60,29 @@ def refine(rho, backend, steps=20, lr=1e-2, kind="poisson", eps=1e-3):
         opt.step()
         hist.append(loss.detach().cpu().numpy())
     return x.detach(), hist
+
+
+def refine_support(rho, support, backend, steps=20, lr_rel=1e-2, kind="poisson", eps=1e-3):
+    """Support-projected Adam refinement (the GRAD operation of the memetic loop)
+    written with torch.optim.Adam: one parameter group per field with step
+    lr_rel * RMS(rho on its support), and the gradient masked by the support.
+    rho: complex64 [P, N, N] on the GPU; support: 0/1 [P, N, N]."""
+    x = rho.detach().clone()
+    S = torch.as_tensor(support, device=x.device).to(torch.float32)
+    params, groups = [], []
+    for p in range(x.shape[0]):
+        sp = S[p] > 0
+        cnt = int(sp.sum().item())
+        e = float((x[p][sp].abs().double() ** 2).sum().item()) if cnt else 0.0
+        lr = lr_rel * (e / cnt) ** 0.5 if cnt else 0.0
+        t = x[p].clone().requires_grad_(True)
+        t.register_hook(lambda g, m=S[p]: g * m)
+        params.append(t)
+        groups.append({"params": [t], "lr": lr})
+    opt = torch.optim.Adam(groups)
+    for _ in range(steps):
+        opt.zero_grad()
+        loss = fourier_loss(torch.stack(params), backend, kind, eps)
+        loss.sum().backward()
+        opt.step()
+    return torch.stack([t.detach() for t in params])

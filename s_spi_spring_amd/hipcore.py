@@ -60,6 +60,7 @@ SIGNATURES = {
     "spr_device_name": (_I, [_I, ctypes.c_char_p, _I]),
     "spr_loss_grad": (_I, [_VP, _I, _VP, _VP, _VP, _I, _F, _VP]),
     "spr_refine": (_I, [_VP, _I, _F, _I, _F, _VP]),
+    "spr_refine_support": (_I, [_VP, _I, _F, _I, _F, _VP]),
 }
 
 
@@ -295,6 +296,11 @@ class Backend:
     def refine(self, steps, lr=1e-2, kind=0, eps=1e-3):
         loss = np.zeros(self.P, np.float64)
         self._check(self.lib.spr_refine(self.h, int(steps), float(lr), int(kind), float(eps), _ptr(loss)))
+        return loss
+
+    def refine_support(self, steps, lr_rel=1e-2, kind=0, eps=1e-3):
+        loss = np.zeros(self.P, np.float64)
+        self._check(self.lib.spr_refine_support(self.h, int(steps), float(lr_rel), int(kind), float(eps), _ptr(loss)))
         return loss
 
     def timing(self, enable: bool, stride: int = 1):

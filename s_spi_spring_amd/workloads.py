@@ -9,8 +9,10 @@ plain_512     512^2 ellipsoids: RAAR 400 (beta 0.87) + HIO 100 (beta 0.95) + ER 
               sigma 3 -> 1.5, threshold SW_THRESHOLD_512 = 0.03, the reference's
               default shrink-wrap threshold, spring/settings_manager.py:119;
               at 0.2 the support of this object collapses within a generation).
-grad_512      the same blocks with a 20-step Adam refinement of the masked Poisson
-              likelihood (lr 1e-2) after each RAAR block.
+grad_512      the same blocks with a support-projected Adam refinement of the masked
+              Poisson likelihood after each RAAR block (GRAD: 10 steps, step 3e-2
+              of the RMS density on the support; of 20/10 steps, 1e-2/3e-2 and the
+              Poisson/amplitude losses this reached E_F < 0.05 soonest on 4 seeds).
 
 time_to_rf runs one memetic reconstruction of the 512^2 ellipsoid pattern with
 one of these schedules and reports the wall-clock time from population
@@ -39,10 +41,10 @@ def plain_small(generation=0, area=None):
 SW_THRESHOLD_512 = 0.03
 
 
-def _blocks_512(grad):
+def _blocks_512(grad, steps=10, lr=3e-2, kind=0):
     blocks = []
     for _ in range(16):
-        blocks.append([("RAAR", [25, 0.87])] + ([("GRAD", [20, 1e-2])] if grad else []))
+        blocks.append([("RAAR", [25, 0.87])] + ([("GRAD", [steps, lr, kind])] if grad else []))
     blocks += [[("HIO", [25, 0.95])]] * 4 + [[("ER", [25])]] * 2
     return blocks
 
@@ -56,8 +58,8 @@ def plain_512(generation=0, area=None, threshold=None):
     return seq
 
 
-def grad_512(generation=0, area=None, threshold=None):
-    blocks = _blocks_512(True)
+def grad_512(generation=0, area=None, threshold=None, steps=10, lr=3e-2, kind=0):
+    blocks = _blocks_512(True, steps, lr, kind)
     th = SW_THRESHOLD_512 if threshold is None else threshold
     seq = []
     for k, b in enumerate(blocks):
@@ -92,7 +94,8 @@ def noise_floor(I, truth):
     return float(np.sqrt(((A[k] - s * F[k]) ** 2).sum() / (A[k] ** 2).sum()))
 
 
-def time_to_rf(seed, form="plain", target=0.05, generations=10, popsize=64, n=512, photons=1e9, threshold=None):
+def time_to_rf(seed, form="plain", target=0.05, generations=10, popsize=64, n=512, photons=1e9, threshold=None,
+               schedule=None):
     """One memetic run on the ellipsoid pattern of `seed`; returns the time to
     best E_F < target (None when not reached within `generations`), the
     generation reached, the final Fourier and real-space errors of the best
@@ -107,7 +110,7 @@ def time_to_rf(seed, form="plain", target=0.05, generations=10, popsize=64, n=51
     s.set_advanced("seed", 1000 + seed)
     m = MPR(Pattern.from_intensity(I), s)
     t0 = time.perf_counter()
-    sched = SCHEDULES[form]
+    sched = SCHEDULES[form] if schedule is None else schedule
     res = m.run(target_error=target, log=False,
                 schedule=lambda g, area: sched(g, area, threshold=threshold))
     wall = time.perf_counter() - t0

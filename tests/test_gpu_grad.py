@@ -91,3 +91,20 @@ def test_device_adam_matches_torch_adam(hip):
     got = be.get_population(support=False)
     assert rel_l2(got, ref.cpu().numpy()) < 1e-4
     assert np.all(loss < hist[0])   # the refinement lowers the loss
+
+
+def test_support_refinement_matches_torch_adam(hip):
+    """spr_refine_support (the GRAD operation) = torch.optim.Adam with one group
+    per field, lr = lr_rel x RMS density on the support, gradient masked by it."""
+    import torch
+    from s_spi_spring_amd.grad import refine_support
+    amp, rho, supp = case(128, 3, seed=24)
+    supp[2] = 0                                     # an empty support: nothing moves
+    be = hip.Backend(amp, capacity=3)
+    ref = refine_support(torch.from_numpy(rho).cuda(), supp, be, steps=10, lr_rel=2e-2).cpu().numpy()
+    be.set_population(rho, supp)
+    be.refine_support(10, lr_rel=2e-2)
+    got = be.get_population(support=False)
+    assert rel_l2(got[:2], ref[:2]) < 1e-4
+    assert np.array_equal(got[supp == 0], rho[supp == 0])      # outside the supports: unchanged
+    assert not np.array_equal(got[0][supp[0] == 1], rho[0][supp[0] == 1])
