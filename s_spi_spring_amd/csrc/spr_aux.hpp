@@ -166,19 +166,19 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row_aux(AuxRowArgs a) {
     if constexpr (MODE == AROW_ABS_FWD_WORK) {
 #pragma unroll
         for (int r = 0; r < 8; ++r) v[r] = a.work[base + t + r * T];
-        fft_line<N, -1>(v, acc, tw, t);
+        fft_line<N, -1, (G::T <= 64)>(v, acc, tw, t);
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const float x = v[q].x * a.inv_nn, yv = v[q].y * a.inv_nn;
             v[q] = make_float2(sqrtf(x * x + yv * yv), 0.f);
         }
-        fft_line<N, +1>(v, acc, tw, t);
+        fft_line<N, +1, (G::T <= 64)>(v, acc, tw, t);
 #pragma unroll
         for (int q = 0; q < 8; ++q) a.work[base + t + q * T] = v[q];
     } else if constexpr (MODE == AROW_MAG_MAX) {
 #pragma unroll
         for (int r = 0; r < 8; ++r) v[r] = a.work[base + t + r * T];
-        fft_line<N, -1>(v, acc, tw, t);
+        fft_line<N, -1, (G::T <= 64)>(v, acc, tw, t);
         float m = 0.f;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
@@ -202,13 +202,13 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row_aux(AuxRowArgs a) {
         }
         block_sum<1, NT>(ms, red);
         if (tid == 0) a.partial[blockIdx.x] = ms[0];
-        fft_line<N, +1>(v, acc, tw, t);
+        fft_line<N, +1, (G::T <= 64)>(v, acc, tw, t);
 #pragma unroll
         for (int q = 0; q < 8; ++q) a.work[base + t + q * T] = v[q];
     } else if constexpr (MODE == AROW_ERR_REAL) {
 #pragma unroll
         for (int r = 0; r < 8; ++r) v[r] = a.work[base + t + r * T];
-        fft_line<N, -1>(v, acc, tw, t);
+        fft_line<N, -1, (G::T <= 64)>(v, acc, tw, t);
         double s[2] = {0.0, 0.0};  // error, norm
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row_aux(AuxRowArgs a) {
             const float2 d = a.src[sbase + t + r * T];
             v[r] = make_float2(sqrtf(d.x * d.x + d.y * d.y), 0.f);
         }
-        fft_line<N, +1>(v, acc, tw, t);
+        fft_line<N, +1, (G::T <= 64)>(v, acc, tw, t);
 #pragma unroll
         for (int q = 0; q < 8; ++q) a.work[base + t + q * T] = v[q];
     } else if constexpr (MODE == AROW_CORR) {
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row_aux(AuxRowArgs a) {
             const float2* src = f == 0 ? a.work : a.work2;
 #pragma unroll
             for (int r = 0; r < 8; ++r) v[r] = src[base + t + r * T];
-            fft_line<N, -1>(v, acc, tw, t);
+            fft_line<N, -1, (G::T <= 64)>(v, acc, tw, t);
             unsigned long long best = 0;
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
@@ -528,7 +528,7 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_fft_rows(FieldBatch b) 
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 8; ++r) v[r] = s[t + r * T];
-    fft_line<N, SIGN>(v, acc, tw, t);
+    fft_line<N, SIGN, (G::T <= 64)>(v, acc, tw, t);
 #pragma unroll
     for (int q = 0; q < 8; ++q) f[t + q * T] = make_float2(v[q].x * b.scale, v[q].y * b.scale);
 }

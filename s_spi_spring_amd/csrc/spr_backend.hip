@@ -601,6 +601,42 @@ template <int N> int launch_col_loss(spr_ctx* c, LossArgs a, int items) {
 }
 int col_loss(spr_ctx* c, const LossArgs& a, int items) { SPR_DISPATCH_N(c, launch_col_loss, c, a, items); }
 
+template <int N> int launch_row_adam(spr_ctx* c, const AdamRowArgs& a, bool masked, int items) {
+    const dim3 grid((unsigned)((size_t)items * N / Geo<N>::RW)), block(Geo<N>::ROW_THREADS);
+    if (masked) hipLaunchKernelGGL((k_row_adam<N, true>), grid, block, 0, c->stream, a);
+    else hipLaunchKernelGGL((k_row_adam<N, false>), grid, block, 0, c->stream, a);
+    SPR_HIP(c, hipGetLastError());
+    return SPR_OK;
+}
+int row_adam(spr_ctx* c, const AdamRowArgs& a, bool masked, int items) {
+    SPR_DISPATCH_N(c, launch_row_adam, c, a, masked, items);
+}
+
+// `steps` Adam steps of the Fourier loss on the resident densities, two passes
+// per step: the column loss pass and the fused adjoint-row / Adam / row-FFT pass.
+int queue_adam_steps(spr_ctx* c, int steps, float lr, const float* lrp, int kind, float eps) {
+    const int P = c->P;
+    const size_t n = (size_t)c->n * c->n * P;
+    SPR_TRY2(ensure(c, c->adam_m, n * sizeof(float2)));
+    SPR_TRY2(ensure(c, c->adam_v, n * sizeof(float2)));
+    SPR_TRY2(fill_u32(c, (unsigned*)c->adam_m.p, 0u, (int)(2 * n)));
+    SPR_TRY2(fill_u32(c, (unsigned*)c->adam_v.p, 0u, (int)(2 * n)));
+    FieldBatch fwd{c->work, nullptr, c->tw, 1.f, c->rho};
+    SPR_TRY2(fft_rows(c, +1, fwd, P));
+    const float b1 = 0.9f, b2 = 0.999f;
+    float p1 = 1.f, p2 = 1.f;
+    for (int s = 1; s <= steps; ++s) {
+        LossArgs la{c->work, c->amp, c->tw, c->partial, eps, kind};
+        SPR_TRY2(col_loss(c, la, P));
+        p1 *= b1;
+        p2 *= b2;
+        AdamRowArgs ar{c->rho, (float2*)c->adam_m.p, (float2*)c->adam_v.p, c->work, c->supp, lrp, c->tw,
+                       lr, b1, b2, 1e-8f, 1.f - p1, 1.f - p2};
+        SPR_TRY2(row_adam(c, ar, lrp != nullptr, P));
+    }
+    return SPR_OK;
+}
+
 // Runs the handle's launches on another stream for the lifetime of the guard.
 struct StreamSwap {
     spr_ctx* c;
@@ -622,6 +658,17 @@ int queue_loss_grad(spr_ctx* c, int P, const float2* rho, float2* grad, int kind
     SPR_TRY2(reduce(c, n / cols_per_wg(n), 1, 7, P));
     FieldBatch adj{grad, nullptr, c->tw, 2.f, c->work};
     return fft_rows(c, -1, adj, P);
+}
+
+// Loss of every resident density after a refinement (host, P doubles).
+int final_loss(spr_ctx* c, int kind, float eps, double* loss) {
+    const int P = c->P;
+    SPR_TRY2(queue_loss_grad(c, P, c->rho, c->work2, kind, eps));
+    std::vector<double> sm((size_t)P * SUMS_PER_ITEM);
+    SPR_HIP(c, hipMemcpyAsync(sm.data(), c->sums, sm.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    for (int p = 0; p < P; ++p) loss[p] = sm[(size_t)p * SUMS_PER_ITEM + 7];
+    return SPR_OK;
 }
 
 int check_indices(spr_ctx* c, const int32_t* idx, int count, int limit, const char* what) {
@@ -1363,30 +1410,9 @@ int spr_refine(spr_ctx* c, int steps, float lr, int kind, float eps, double* los
     if (kind != LOSS_POISSON && kind != LOSS_AMPLITUDE) return fail(c, SPR_ERR_ARG, "unknown loss kind");
     const int P = c->P;
     if (P <= 0 || steps == 0) return SPR_OK;
-    const size_t n = (size_t)c->n * c->n * P;
     SPR_HIP(c, hipSetDevice(c->device));
-    SPR_TRY(ensure(c, c->adam_m, n * sizeof(float2)));
-    SPR_TRY(ensure(c, c->adam_v, n * sizeof(float2)));
-    SPR_TRY(fill_u32(c, (unsigned*)c->adam_m.p, 0u, (int)(2 * n)));
-    SPR_TRY(fill_u32(c, (unsigned*)c->adam_v.p, 0u, (int)(2 * n)));
-    const float b1 = 0.9f, b2 = 0.999f, aeps = 1e-8f;
-    float p1 = 1.f, p2 = 1.f;
-    for (int s = 1; s <= steps; ++s) {
-        SPR_TRY(queue_loss_grad(c, P, c->rho, c->work2, kind, eps));
-        p1 *= b1;
-        p2 *= b2;
-        hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->rho,
-                           (const float2*)c->work2, (float2*)c->adam_m.p, (float2*)c->adam_v.p, n, lr, b1, b2, aeps,
-                           1.f - p1, 1.f - p2);
-        SPR_HIP(c, hipGetLastError());
-    }
-    if (loss) {
-        SPR_TRY(queue_loss_grad(c, P, c->rho, c->work2, kind, eps));
-        std::vector<double> sm((size_t)P * SUMS_PER_ITEM);
-        SPR_HIP(c, hipMemcpyAsync(sm.data(), c->sums, sm.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        SPR_HIP(c, hipStreamSynchronize(c->stream));
-        for (int p = 0; p < P; ++p) loss[p] = sm[(size_t)p * SUMS_PER_ITEM + 7];
-    }
+    SPR_TRY(queue_adam_steps(c, steps, lr, nullptr, kind, eps));
+    if (loss) SPR_TRY(final_loss(c, kind, eps, loss));
     SPR_HIP(c, hipStreamSynchronize(c->stream));
     return harvest_timing(c);
 }
@@ -1398,35 +1424,14 @@ int spr_refine_support(spr_ctx* c, int steps, float lr_rel, int kind, float eps,
     const int P = c->P;
     if (P <= 0 || steps == 0) return SPR_OK;
     const int np = c->n * c->n;
-    const size_t n = (size_t)np * P;
     SPR_HIP(c, hipSetDevice(c->device));
-    SPR_TRY(ensure(c, c->adam_m, n * sizeof(float2)));
-    SPR_TRY(ensure(c, c->adam_v, n * sizeof(float2)));
     SPR_TRY(ensure(c, c->xphase, (size_t)P * sizeof(float)));
-    SPR_TRY(fill_u32(c, (unsigned*)c->adam_m.p, 0u, (int)(2 * n)));
-    SPR_TRY(fill_u32(c, (unsigned*)c->adam_v.p, 0u, (int)(2 * n)));
     float* lrp = (float*)c->xphase.p;
     hipLaunchKernelGGL(k_support_scale, dim3(P), dim3(256), 0, c->stream, (const float2*)c->rho,
                        (const uint8_t*)c->supp, np, lr_rel, lrp);
     SPR_HIP(c, hipGetLastError());
-    const float b1 = 0.9f, b2 = 0.999f, aeps = 1e-8f;
-    float p1 = 1.f, p2 = 1.f;
-    for (int s = 1; s <= steps; ++s) {
-        SPR_TRY(queue_loss_grad(c, P, c->rho, c->work2, kind, eps));
-        p1 *= b1;
-        p2 *= b2;
-        hipLaunchKernelGGL(k_adam_support, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->rho,
-                           (const float2*)c->work2, (float2*)c->adam_m.p, (float2*)c->adam_v.p,
-                           (const uint8_t*)c->supp, (const float*)lrp, np, n, b1, b2, aeps, 1.f - p1, 1.f - p2);
-        SPR_HIP(c, hipGetLastError());
-    }
-    if (loss) {
-        SPR_TRY(queue_loss_grad(c, P, c->rho, c->work2, kind, eps));
-        std::vector<double> sm((size_t)P * SUMS_PER_ITEM);
-        SPR_HIP(c, hipMemcpyAsync(sm.data(), c->sums, sm.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        SPR_HIP(c, hipStreamSynchronize(c->stream));
-        for (int p = 0; p < P; ++p) loss[p] = sm[(size_t)p * SUMS_PER_ITEM + 7];
-    }
+    SPR_TRY(queue_adam_steps(c, steps, 0.f, lrp, kind, eps));
+    if (loss) SPR_TRY(final_loss(c, kind, eps, loss));
     SPR_HIP(c, hipStreamSynchronize(c->stream));
     return harvest_timing(c);
 }

@@ -42,14 +42,18 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_loss(LossArgs a) {
     const int off = t * N + c;
     const ColAcc<G::CS> acc{lds + c};
     float2 v[8];
+    float Aq[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = f[off + r * T * N];
+    for (int r = 0; r < 8; ++r) {
+        v[r] = f[off + r * T * N];
+        Aq[r] = amp[off + r * T * N];
+    }
     __syncthreads();
     fft_line<N, +1>(v, acc, tw, t);
     double l[1] = {0.0};
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-        const float A = amp[off + q * T * N];
+        const float A = Aq[q];
         const float u = v[q].x * v[q].x + v[q].y * v[q].y;
         float w = 0.f;
         if (A >= 0.f) {
@@ -71,24 +75,6 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_loss(LossArgs a) {
     fft_line<N, -1>(v, acc, tw, t);
 #pragma unroll
     for (int q = 0; q < 8; ++q) f[off + q * T * N] = v[q];
-}
-
-// Adam step on complex fields viewed as real pairs (torch.optim.Adam semantics,
-// no weight decay): m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;
-// x -= lr * (m / (1-b1^t)) / (sqrt(v / (1-b2^t)) + eps).
-__global__ __launch_bounds__(256) void k_adam(float2* x, const float2* g, float2* m, float2* v, size_t n, float lr,
-                                              float b1, float b2, float eps, float bc1, float bc2) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    const float2 gi = g[i];
-    float2 mi = m[i], vi = v[i], xi = x[i];
-    mi = make_float2(b1 * mi.x + (1.f - b1) * gi.x, b1 * mi.y + (1.f - b1) * gi.y);
-    vi = make_float2(b2 * vi.x + (1.f - b2) * gi.x * gi.x, b2 * vi.y + (1.f - b2) * gi.y * gi.y);
-    xi.x -= lr * (mi.x / bc1) / (sqrtf(vi.x / bc2) + eps);
-    xi.y -= lr * (mi.y / bc1) / (sqrtf(vi.y / bc2) + eps);
-    m[i] = mi;
-    v[i] = vi;
-    x[i] = xi;
 }
 
 // Per-individual step size of the support-projected refinement:
@@ -119,24 +105,69 @@ __global__ __launch_bounds__(256) void k_support_scale(const float2* rho, const 
     if (threadIdx.x == 0) lrp[p] = sc[0] > 0 ? (float)(lr_rel * sqrt(se[0] / sc[0])) : 0.f;
 }
 
-// Adam step restricted to the support: pixels outside it keep their value (and
-// zero moments), as torch.optim.Adam does for a gradient masked by the support;
-// the step size of individual p is lrp[p].
-__global__ __launch_bounds__(256) void k_adam_support(float2* x, const float2* g, float2* m, float2* v,
-                                                      const uint8_t* supp, const float* lrp, int np, size_t n,
-                                                      float b1, float b2, float eps, float bc1, float bc2) {
-    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n || !supp[i]) return;
-    const float lr = lrp[i / np];
-    const float2 gi = g[i];
-    float2 mi = m[i], vi = v[i], xi = x[i];
-    mi = make_float2(b1 * mi.x + (1.f - b1) * gi.x, b1 * mi.y + (1.f - b1) * gi.y);
-    vi = make_float2(b2 * vi.x + (1.f - b2) * gi.x * gi.x, b2 * vi.y + (1.f - b2) * gi.y * gi.y);
-    xi.x -= lr * (mi.x / bc1) / (sqrtf(vi.x / bc2) + eps);
-    xi.y -= lr * (mi.y / bc1) / (sqrtf(vi.y / bc2) + eps);
-    m[i] = mi;
-    v[i] = vi;
-    x[i] = xi;
+// Row pass of one refinement step, fused: the adjoint row transform of the
+// column pass's output gives the gradient g = 2 FT^H[f'(u) F] of each row; an
+// Adam step (torch.optim.Adam semantics, no weight decay: m = b1 m + (1-b1) g;
+// v = b2 v + (1-b2) g^2; x -= lr (m/(1-b1^t)) / (sqrt(v/(1-b2^t)) + eps), on
+// the real and imaginary parts) updates the densities and the moments; the row FFT of the updated densities goes back to
+// `work` for the next step's column pass. MASKED: only support pixels move and
+// the step of field p is lrp[p]; otherwise every pixel moves by lr.
+struct AdamRowArgs {
+    float2* x;
+    float2* m;
+    float2* v;
+    float2* work;
+    const uint8_t* supp;
+    const float* lrp;
+    const float2* tw;
+    float lr, b1, b2, eps, bc1, bc2;
+};
+
+template <int N, bool MASKED>
+__global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row_adam(AdamRowArgs a) {
+    using G = Geo<N>;
+    constexpr int T = G::T;
+    constexpr bool WV = T <= 64;
+    __shared__ float2 tw[TwTable<N>::SIZE];
+    __shared__ float2 lds[G::RW * G::RS];
+    const int tid = threadIdx.x;
+    const int t = tid % T;
+    const int rl = tid / T;
+    tw_fill<N>(tw, a.tw, tid, G::ROW_THREADS);
+    const size_t blk = (size_t)blockIdx.x * G::RW * N;
+    const int field = (int)(blk / ((size_t)N * N));
+    const int off = rl * N + t;
+    float2* __restrict__ x = a.x + blk;
+    float2* __restrict__ m = a.m + blk;
+    float2* __restrict__ v2 = a.v + blk;
+    float2* __restrict__ work = a.work + blk;
+    const RowAcc acc{lds + rl * G::RS};
+    const float lr = MASKED ? a.lrp[field] : a.lr;
+    float2 v[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) v[r] = work[off + r * T];
+    __syncthreads();
+    fft_line<N, -1, WV>(v, acc, tw, t);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int i = off + q * T;
+        const float2 gi = make_float2(v[q].x * 2.f, v[q].y * 2.f);
+        float2 xi = x[i];
+        if (!MASKED || a.supp[blk + i]) {
+            float2 mi = m[i], vi = v2[i];
+            mi = make_float2(a.b1 * mi.x + (1.f - a.b1) * gi.x, a.b1 * mi.y + (1.f - a.b1) * gi.y);
+            vi = make_float2(a.b2 * vi.x + (1.f - a.b2) * gi.x * gi.x, a.b2 * vi.y + (1.f - a.b2) * gi.y * gi.y);
+            xi.x -= lr * (mi.x / a.bc1) / (sqrtf(vi.x / a.bc2) + a.eps);
+            xi.y -= lr * (mi.y / a.bc1) / (sqrtf(vi.y / a.bc2) + a.eps);
+            m[i] = mi;
+            v2[i] = vi;
+            x[i] = xi;
+        }
+        v[q] = xi;
+    }
+    fft_line<N, +1, WV>(v, acc, tw, t);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) work[off + q * T] = v[q];
 }
 
 }  // namespace spr

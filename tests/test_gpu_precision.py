@@ -1,0 +1,109 @@
+"""Precision at the large sizes (truncated-octahedron patterns): every FP32
+kernel result against the FP64 oracle from the same state, with the error
+stated (printed) and bounded.
+
+1024^2 (1e10 photons) and 2048^2: one FT, one HIO / RAAR / ER iteration,
+shrink-wrap (mask bit-exact outside the rounding band), get_error metrics; and
+at 1024^2 with 1e8 photons (the low-count gradient case) the Poisson gradient
+against the FP64 closed form and central finite differences of the FP64 loss
+(step 1e-3, 64 seeded pixels). Bars as in the smaller-size tests: fields
+relative L2 <= 1e-5, metrics relative <= 1e-5, gradient <= 1e-4.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from s_spi_spring_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def octa_case(n, P, seed, photons=1e10):
+    I, truth = synth.octahedron_pattern(seed=seed, n=n, photons=photons)
+    amp = synth.corner_amplitude(I)
+    rmax = 150 * n // 1024 + 8
+    supp = synth.disk_support(n, 2 * rmax)
+    rho = synth.random_start(n, P, supp, seed=seed + 1, scale=float(np.sqrt(np.maximum(I, 0).sum()) / supp.sum()))
+    return amp, rho, np.broadcast_to(supp, rho.shape).astype(np.int32).copy()
+
+
+@pytest.mark.parametrize("n", [1024, 2048])
+def test_iteration_and_transform_precision(hip, orc, n):
+    amp, rho, supp = octa_case(n, 1, seed=31)
+    amp64 = amp.astype(np.float64)
+    be = hip.Backend(amp, capacity=1)
+    be.set_population(rho, supp)
+    be.ft()
+    e = rel_l2(be.get_population(support=False)[0], orc.ft(rho[0].astype(np.complex128)))
+    print(f"n={n} FT rel-L2 vs FP64 {e:.2e}")
+    assert e < 1e-5
+    for alg, beta in (("HIO", 0.9), ("RAAR", 0.87), ("ER", 0.0)):
+        be.set_population(rho, supp)
+        be.iterate(alg, 1, beta, 0.0)
+        got = be.get_population(support=False)[0]
+        want = orc.iterate(alg, rho[0].astype(np.complex128), supp[0], amp64, 1, beta, 0.0)
+        e = rel_l2(got, want)
+        print(f"n={n} {alg} one iteration rel-L2 vs FP64 {e:.2e}")
+        assert e < 1e-5
+
+
+@pytest.mark.parametrize("n", [1024, 2048])
+def test_shrinkwrap_and_error_precision(hip, orc, n):
+    amp, rho, supp = octa_case(n, 1, seed=32)
+    amp64 = amp.astype(np.float64)
+    rho = orc.iterate("ER", rho, supp, amp, 2, 0.0, 0.0)
+    be = hip.Backend(amp, capacity=1)
+    be.set_population(rho, supp)
+    ev = be.evaluate(0.0)
+    ef, er_, cs, ms = orc.get_error(rho[0].astype(np.complex128), supp[0], amp64, 0.0)
+    rel = [abs(ev[k][0] - w) / abs(w) for k, w in (("error_fourier", ef), ("complexsum", cs), ("modulesum", ms))]
+    print(f"n={n} get_error relative errors vs FP64 (E_F, max|F|, sum|rho|): " + ", ".join(f"{r:.1e}" for r in rel))
+    assert max(rel) < 1e-5
+    be.sw(2.0, 0.1)
+    got_S = be.get_population()[1][0]
+    b = np.abs(orc.blur(rho[0].astype(np.complex128), amp64, 2.0))
+    want = orc.sw(rho[0].astype(np.complex128), supp[0], amp64, 2.0, 0.1)
+    thr = 0.1 * b.max()
+    band = np.abs(b - thr) <= 1e-5 * thr
+    diff = got_S != want
+    print(f"n={n} SW: {diff.sum()} differing pixels, {band.sum()} in the rounding band")
+    assert (diff & ~band).sum() == 0 and band.sum() <= 4
+
+
+def test_low_count_gradient_1024(hip):
+    import torch
+    n = 1024
+    amp, rho, supp = octa_case(n, 1, seed=33, photons=1e8)
+    be = hip.Backend(amp, capacity=1)
+    x = torch.from_numpy(rho).cuda()
+    g = torch.empty_like(x)
+    loss = be.loss_grad(1, x.data_ptr(), g.data_ptr(), 0, 1e-3, torch.cuda.current_stream().cuda_stream)
+    g = g.cpu().numpy()[0]
+    F = np.fft.ifft2(rho[0].astype(np.complex128)) * n * n
+    A = amp.astype(np.float64)
+    k = A >= 0
+    u = np.abs(F) ** 2
+    I = A ** 2
+    L = np.sum((u - I * np.log(u + 1e-3))[k])
+    gref = 2 * np.fft.fft2(np.where(k, 1 - I / (u + 1e-3), 0) * F)
+    e = rel_l2(g, gref)
+    print(f"n=1024 low-count Poisson gradient rel-L2 vs FP64 {e:.2e}, loss rel {abs(loss[0] - L) / abs(L):.1e}")
+    assert e < 1e-4 and abs(loss[0] - L) <= 1e-6 * abs(L)
+
+    def loss64(r):
+        Fr = np.fft.ifft2(r) * n * n
+        ur = np.abs(Fr) ** 2
+        return np.sum((ur - I * np.log(ur + 1e-3))[k])
+    rng = np.random.default_rng(7)
+    inside = np.flatnonzero(supp[0].ravel())
+    h, scale = 1e-3, np.abs(gref).max()
+    r0 = rho[0].astype(np.complex128)
+    for p in rng.choice(inside, 64, replace=False):
+        y, xx = divmod(int(p), n)
+        for d in (1.0, 1j):
+            r1, r2 = r0.copy(), r0.copy()
+            r1[y, xx] += h * d
+            r2[y, xx] -= h * d
+            fd = (loss64(r1) - loss64(r2)) / (2 * h)
+            got = g[y, xx].real if d == 1.0 else g[y, xx].imag
+            assert abs(fd - got) <= 1e-3 * scale, (p, d, fd, got)
