@@ -25,6 +25,7 @@ struct spr_ctx {
     int lanes = 1;
     hipStream_t lane2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+
     float2* rho = nullptr;
     float2* work = nullptr;
     float2* work2 = nullptr;      // second half-transformed field per individual (correlations, |b|)
@@ -469,6 +470,19 @@ int queue_iterate_lanes(spr_ctx* c, int mode, int niter, const RowArgs& a) {
     return SPR_OK;
 }
 
+// One iteration block on the context's stream: the first row transform, niter
+// x [column pass, row update], the trailing projection and the last row pass.
+int queue_iterate_block(spr_ctx* c, int mode, int niter, const RowArgs& a) {
+    SPR_TRY(row(c, ROW_FWD, a));
+    for (int i = 0; i < niter; ++i) {
+        SPR_TRY(col(c, COL_PROJ, c->work));
+        SPR_TRY(row(c, mode, a));
+    }
+    SPR_TRY(col(c, COL_PROJ, c->work));
+    SPR_TRY(row(c, ROW_FINAL, a));
+    return SPR_OK;
+}
+
 int queue_iterate(spr_ctx* c, int alg, int niter, float beta, float reality) {
     if (alg < SPR_ER || alg > SPR_RAAR) return fail(c, SPR_ERR_ARG, "unknown algorithm");
     if (niter < 0) return fail(c, SPR_ERR_ARG, "negative iteration count");
@@ -479,14 +493,7 @@ int queue_iterate(spr_ctx* c, int alg, int niter, float beta, float reality) {
     const int mode = (alg == SPR_HIO) ? ROW_HIO : (alg == SPR_RAAR) ? ROW_RAAR : ROW_ER;
     SPR_HIP(c, hipSetDevice(c->device));
     if (c->lanes > 1 && c->P >= 2 && !c->timing) return queue_iterate_lanes(c, mode, niter, a);
-    SPR_TRY(row(c, ROW_FWD, a));
-    for (int i = 0; i < niter; ++i) {
-        SPR_TRY(col(c, COL_PROJ, c->work));
-        SPR_TRY(row(c, mode, a));
-    }
-    SPR_TRY(col(c, COL_PROJ, c->work));
-    SPR_TRY(row(c, ROW_FINAL, a));
-    return SPR_OK;
+    return queue_iterate_block(c, mode, niter, a);
 }
 
 // ---------------------------------------------------------------------------

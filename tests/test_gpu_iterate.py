@@ -107,3 +107,4 @@ def test_sampled_timing_counts(hip):
     be.timing(False)
     be.iterate("HIO", 2, 0.9, 0.0)
     assert be.timing_read(hip.KIND_ROW_UPDATE)[1] == 0
+
