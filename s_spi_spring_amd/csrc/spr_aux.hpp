@@ -300,8 +300,9 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_aux(AuxColArgs a) {
     const int t = tid / C;
     tw_fill<N>(tw, a.tw, tid, NT);
     constexpr int TILES = N / C;
-    const int item = blockIdx.x / TILES;
-    const int x0 = (blockIdx.x % TILES) * C;
+    const int strip = col_strip<N>(blockIdx.x);
+    const int item = strip / TILES;
+    const int x0 = (strip % TILES) * C;
     const size_t base = (size_t)item * N * N + x0 + c;
     const ColAcc<G::CS> acc{lds + c};
     float2 v[8];
@@ -336,8 +337,8 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_aux(AuxColArgs a) {
         }
         block_sum<2, NT>(s, red);
         if (tid == 0) {
-            a.partial[2 * blockIdx.x] = s[0];
-            a.partial[2 * blockIdx.x + 1] = s[1];
+            a.partial[2 * strip] = s[0];
+            a.partial[2 * strip + 1] = s[1];
         }
         fmax = block_max<NT>(fmax, reinterpret_cast<float*>(red));
         if (tid == 0) atomicMax(&a.maxbits[item], __float_as_uint(fmax));
@@ -542,8 +543,9 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_fft_cols(FieldBatch b) 
     __shared__ float2 lds[N * G::CS];
     const int tid = threadIdx.x, c = tid % C, t = tid / C;
     tw_fill<N>(tw, b.tw, tid, NT);
-    const int item = blockIdx.x / TILES;
-    const int x0 = (blockIdx.x % TILES) * C;
+    const int strip = col_strip<N>(blockIdx.x);
+    const int item = strip / TILES;
+    const int x0 = (strip % TILES) * C;
     float2* __restrict__ f = b.base + field_of(b, item) * N * N + x0;
     const int off = t * N + c;
     const ColAcc<G::CS> acc{lds + c};
@@ -586,8 +588,9 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_corr_spec_cols(CorrSpec
     __shared__ float2 lds[N * G::CS];
     const int tid = threadIdx.x, c = tid % C, t = tid / C;
     tw_fill<N>(tw, a.tw, tid, NT);
-    const int item = blockIdx.x / TILES;
-    const int x0 = (blockIdx.x % TILES) * C;
+    const int strip = col_strip<N>(blockIdx.x);
+    const int item = strip / TILES;
+    const int x0 = (strip % TILES) * C;
     const float2* __restrict__ d = a.spec + (size_t)a.d_index[item] * N * N + x0;
     const float2* __restrict__ r = a.spec + (size_t)a.r_index[item] * N * N + x0;
     const float* __restrict__ amp = a.amp + x0;

@@ -35,8 +35,9 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_loss(LossArgs a) {
     __shared__ double red[4 * (NT / 64)];
     const int tid = threadIdx.x, c = tid % C, t = tid / C;
     tw_fill<N>(tw, a.tw, tid, NT);
-    const int item = blockIdx.x / TILES;
-    const int x0 = (blockIdx.x % TILES) * C;
+    const int strip = col_strip<N>(blockIdx.x);
+    const int item = strip / TILES;
+    const int x0 = (strip % TILES) * C;
     float2* __restrict__ f = a.f + (size_t)item * N * N + x0;
     const float* __restrict__ amp = a.amp + x0;
     const int off = t * N + c;
@@ -71,7 +72,7 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_loss(LossArgs a) {
         v[q] = make_float2(v[q].x * w, v[q].y * w);
     }
     block_sum<1, NT>(l, red);
-    if (tid == 0) a.partial[blockIdx.x] = l[0];
+    if (tid == 0) a.partial[strip] = l[0];
     fft_line<N, -1>(v, acc, tw, t);
 #pragma unroll
     for (int q = 0; q < 8; ++q) f[off + q * T * N] = v[q];

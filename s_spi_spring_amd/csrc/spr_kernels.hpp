@@ -37,6 +37,9 @@ template <int N> struct Geo {
 #ifndef SPR_COL_WAVE
 #define SPR_COL_WAVE 0         // 1: wavefront-owned columns when N/8 <= 64
 #endif
+#ifndef SPR_COL_XCD
+#define SPR_COL_XCD 1          // XCD-aware strip order for strips narrower than 128 bytes
+#endif
 #ifndef SPR_COL_AMP_PREFETCH
 #define SPR_COL_AMP_PREFETCH 1 // load the amplitude with the field, before the forward transform
 #endif
@@ -306,6 +309,23 @@ struct ColArgs {
     const float* amp_t; // the amplitude transposed, amp_t[x*N + y] = amp[y*N + x]
 };
 
+// Column strip of workgroup `b`. Workgroups are dispatched round-robin over the
+// 8 XCDs (b mod 8), each with its own L2. When a strip is narrower than a
+// 128-byte line (C < 16, N >= 1024), the 16/C strips that share those lines
+// are given to workgroups b, b+8, ... of the same XCD, so each line is fetched
+// from HBM into one L2 only.
+template <int N> __device__ __forceinline__ int col_strip(int b) {
+    constexpr int C = Geo<N>::C;
+    constexpr int GS = C < 16 ? 16 / C : 1;     // strips per 128-byte line
+    if constexpr (GS == 1 || SPR_COL_XCD == 0) {
+        return b;
+    } else {
+        constexpr int CH = 8 * GS;              // workgroups per chunk of lines
+        const int j = b % CH;
+        return (b - j) + (j % 8) * GS + j / 8;
+    }
+}
+
 // Column pass over C adjacent columns of one field. Global memory is read and
 // written as rows of C consecutive values (coalesced); the C columns' transforms
 // exchange data through a padded LDS tile with workgroup barriers. With
@@ -326,8 +346,9 @@ void k_col(ColArgs a) {
     const int t = tid / C;
     tw_fill<N>(tw, a.tw, tid, G::COL_THREADS);
     constexpr int TILES = N / C;
-    const int p = blockIdx.x / TILES;
-    const int x0 = (blockIdx.x % TILES) * C;
+    const int strip = col_strip<N>(blockIdx.x);
+    const int p = strip / TILES;
+    const int x0 = (strip % TILES) * C;
     // Block-uniform bases with 32-bit per-thread offsets (SGPR base addressing).
     float2* __restrict__ f = a.f + (size_t)p * N * N + x0;
     const int off = t * N + c;
