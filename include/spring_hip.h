@@ -202,8 +202,10 @@ typedef struct spr_cross_desc {
 /* prepare_parents + get_new_individual (src/gpu/gpu_context.cu:1661-1694,
  * 2044-2117) for nchild children at once. avg_rho: the population average
  * density (n*n complex); amp_avg: GetAverageModule(ft, 0.66) (n*n floats).
- * Outputs: child densities (nchild*n*n complex) and supports (nchild*n*n int32;
- * with update_support == 0 they are p0's supports). */
+ * Outputs: child densities (nchild*n*n complex) and the crossed supports
+ * (nchild*n*n int32). update_support is accepted for signature parity and not
+ * used here: as in the reference (src/solver.cpp:810), the caller keeps p0's
+ * support when it is 0. */
 int spr_crossover(spr_ctx* ctx, int nchild, const spr_cross_desc* desc, const int32_t* crossmaps, int crossmap_len,
                   const float* avg_rho, const float* amp_avg, float crossdiff, float use_average, float minr,
                   float maxr, float reality, int update_support, float* child_rho, int32_t* child_supp);

@@ -30,6 +30,7 @@ except ImportError:
 
 from .alg_manager import ER, HIO, RAAR, SW, AlgManager  # noqa: F401
 from .pattern import Pattern  # noqa: F401
+from .result import Result  # noqa: F401
 from .settings import Settings  # noqa: F401
 from .stats import Stats  # noqa: F401
 
@@ -52,7 +53,7 @@ def install_aliases():
     sys.modules.setdefault("spring_core.phase_retrieval", spring_core.phase_retrieval)
     sys.modules.setdefault("spring_core.GPUtils", spring_core.GPUtils)
     spring = types.ModuleType("spring")
-    spring.MPR, spring.Settings, spring.Pattern = MPR, Settings, Pattern
+    spring.MPR, spring.Settings, spring.Pattern, spring.Result = MPR, Settings, Pattern, Result
     spring.Reconstruction = spring_core.phase_retrieval.Reconstruction
     sys.modules.setdefault("spring", spring)
     return spring

@@ -19,7 +19,14 @@ import time
 import numpy as np
 
 from .alg_manager import ER, HIO, RAAR, AlgManager
+from .result import Result
 from .stats import Stats
+
+try:
+    import h5py  # noqa: F401
+    _HAVE_H5PY = True
+except ImportError:       # not in this image: result files are not written
+    _HAVE_H5PY = False
 
 
 class StopRec(Exception):
@@ -115,12 +122,23 @@ class MPR:
     def checkstop(self):
         return self.stopevent is not None and self.stopevent.is_set()
 
-    def save_result(self):
+    def save_result(self, save_all: bool = False):
+        """mpr.py:386-416: statistics of the generation and the Result object."""
         self.stats.add(self.reconstructions, self.bestrec, self.avgrec, self.replcount, self.generation,
                        tstamp=time.time())
-        self.result = {"best_density": self.bestrec.get_density(), "best_support": self.bestrec.get_support(),
-                       "avg_density": self.avgrec.get_density(), "avg_support": self.avgrec.get_support(),
-                       "stats": self.stats.get(), "generation": self.generation}
+        dens_all = [r.get_density() for r in self.reconstructions] if save_all else None
+        supp_all = [r.get_support() for r in self.reconstructions] if save_all else None
+        res = Result()
+        res.set(self.pattern, self.bestrec.get_density(), self.bestrec.get_support(), self.avgrec.get_density(),
+                self.avgrec.get_support(), self.pid, self.stats, self.fullsettings, self.metadata,
+                density_all=dens_all, support_all=supp_all, step=self.generation,
+                total=self.settings["global"]["generations"])
+        self.result = res
+
+    def _save_file(self):
+        """Write the result file (mpr.py:355-356) when h5py is installed."""
+        if _HAVE_H5PY:
+            self.result.save(self.settings["global"]["workdir"], self.tag)
 
     def _record(self, t0):
         best = min(self.reconstructions, key=lambda r: r.get_error())
@@ -143,8 +161,10 @@ class MPR:
                                         update_support_first=update_support_first)
 
     def run(self, save_every: int = -1, target_error: float = None, log: bool = True, schedule=None):
-        """Run the memetic loop; returns the result dictionary (best and average
-        reconstructions and the statistics). With target_error, stop after the
+        """Run the memetic loop; returns the Result (best and average
+        reconstructions, statistics, settings), written to
+        `workdir/<pid>_rec.h5` every `save_every` generations and at the end
+        when h5py is installed. With target_error, stop after the
         first generation whose best Fourier error is below it. `schedule`
         (optional) maps (generation, [best area, average area]) to the
         [(name, [params])] sequence of that generation instead of AlgManager."""
@@ -215,6 +235,8 @@ class MPR:
                     self.reconstructions[j] = child
             self.replcount = int(np.sum(repl))
             self.save_result()
+            if (save_every > 0 and self.generation % save_every == 0) or self.generation == gens:
+                self._save_file()
             self._phase("result", t)
             self._record(t0)
             if log:
@@ -222,5 +244,6 @@ class MPR:
                 self.print(f"gen {self.generation:4d}  best E_F {h['best_fourier_error']:.5f}  "
                            f"replaced {self.replcount}  t {h['time']:.2f} s")
             if self.checkstop():
+                self._save_file()
                 break
         return self.result

@@ -43,5 +43,16 @@ class Stats:
     def get(self):
         return self.stats
 
+    def print(self):
+        """Status block of the last generation (stats_manager.py:105-124)."""
+        g = self.stats["global"]
+        lines = ["\tOS: {:.2f}  Repl: {:.3f}".format(g["OS"][-1], g["replacement"][-1])]
+        for key, label in (("best", "Best\t"), ("mean", "Mean\t"), ("average", "Avg \t")):
+            line = "\t" + label
+            for e, short in (("error", "Err."), ("complexity", "Compl."), ("symmetry", "Sym."), ("module_sum", "Sum.")):
+                line += " " + short + " {:.3e} ;".format(self.stats[key][e][-1])
+            lines.append(line)
+        print("\n".join(lines) + "\n")
+
     def set(self, statsdict):
         self.stats = statsdict

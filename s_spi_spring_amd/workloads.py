@@ -118,7 +118,7 @@ def time_to_rf(seed, form="plain", target=0.05, generations=10, popsize=64, n=51
     reached = h["best_fourier_error"] < target
     return {"seed": seed, "form": form, "time_to_target": h["time"] if reached else None,
             "wall": wall, "generation": h["generation"], "fourier_error": h["best_fourier_error"],
-            "real_space_error": metrics.real_space_error(res["best_density"], truth),
+            "real_space_error": metrics.real_space_error(res.get_density_best(crop=False), truth),
             "history": [round(x["best_fourier_error"], 5) for x in m.history],
             "noise_floor": noise_floor(I, truth),
             "iterations": iterations(sched()) * (h["generation"] + 1)}

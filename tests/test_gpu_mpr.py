@@ -27,8 +27,9 @@ def test_small_memetic_run():
     assert len(h) == 2 and all(np.isfinite(x["best_fourier_error"]) for x in h)
     assert h[-1]["best_fourier_error"] <= h[0]["best_fourier_error"] * 1.0001
     assert h[-1]["best_fourier_error"] < 0.5
-    assert res["best_support"].sum() > 0
-    st = res["stats"]
+    assert res.get_support_best(crop=False).sum() > 0
+    assert res.get_density().shape[0] <= 128 and res.step == 1 and res.total == 1
+    st = res.get_stats()
     assert len(st["best"]["error"]) == 2 and len(st["all"]["error"][0]) == 8
 
 
@@ -38,4 +39,4 @@ def test_memetic_run_is_reproducible():
     e1 = [x["best_fourier_error"] for x in m1.history]
     e2 = [x["best_fourier_error"] for x in m2.history]
     assert e1 == e2
-    assert np.array_equal(r1["best_support"], r2["best_support"])
+    assert np.array_equal(r1.get_support_best(crop=False), r2.get_support_best(crop=False))
