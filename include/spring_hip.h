@@ -50,7 +50,8 @@ enum spr_status {
 
 enum spr_alg { SPR_ER = 0, SPR_HIO = 1, SPR_RAAR = 2 };
 
-/* Supported grid sizes N (spr_supported_size(N) != 0). */
+/* Supported grid sizes N (spr_supported_size(N) != 0): 64, 128, 256, 384, 512,
+ * 768, 1024 and 2048. */
 int spr_supported_size(int n);
 
 /* Device enumeration, replacing GPU_utils::get_count / get_info

@@ -147,7 +147,7 @@ struct AuxRowArgs {
 template <int N, int MODE>
 __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row_aux(AuxRowArgs a) {
     using G = Geo<N>;
-    constexpr int T = G::T, NT = G::ROW_THREADS;
+    constexpr int V = G::V, T = G::T, NT = G::ROW_THREADS;
     constexpr int WG_PER_FIELD = N / G::RW;
     __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[G::RW * G::RS];
@@ -160,28 +160,28 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row_aux(AuxRowArgs a) {
     const int y = (blockIdx.x % WG_PER_FIELD) * G::RW + rl;
     const size_t base = (size_t)item * N * N + (size_t)y * N;
     const RowAcc acc{lds + rl * G::RS};
-    float2 v[8];
+    float2 v[V];
     __syncthreads();
 
     if constexpr (MODE == AROW_ABS_FWD_WORK) {
 #pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = a.work[base + t + r * T];
+        for (int r = 0; r < V; ++r) v[r] = a.work[base + t + r * T];
         fft_line<N, -1, (G::T <= 64)>(v, acc, tw, t);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < V; ++q) {
             const float x = v[q].x * a.inv_nn, yv = v[q].y * a.inv_nn;
             v[q] = make_float2(sqrtf(x * x + yv * yv), 0.f);
         }
         fft_line<N, +1, (G::T <= 64)>(v, acc, tw, t);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) a.work[base + t + q * T] = v[q];
+        for (int q = 0; q < V; ++q) a.work[base + t + q * T] = v[q];
     } else if constexpr (MODE == AROW_MAG_MAX) {
 #pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = a.work[base + t + r * T];
+        for (int r = 0; r < V; ++r) v[r] = a.work[base + t + r * T];
         fft_line<N, -1, (G::T <= 64)>(v, acc, tw, t);
         float m = 0.f;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < V; ++q) {
             const float mm = sqrtf(v[q].y * v[q].y + v[q].x * v[q].x);
             a.mag[base + t + q * T] = mm;
             m = vmax(m, mm);
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row_aux(AuxRowArgs a) {
     } else if constexpr (MODE == AROW_ERR_FWD) {
         double ms[1] = {0.0};
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
+        for (int r = 0; r < V; ++r) {
             const size_t i = base + t + r * T;
             float2 d = a.src[i];
             const float ts = (a.supp[i] != 0 && !phase_fails(d.x, d.y, a.pc)) ? 1.f : 0.f;
@@ -204,14 +204,14 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row_aux(AuxRowArgs a) {
         if (tid == 0) a.partial[blockIdx.x] = ms[0];
         fft_line<N, +1, (G::T <= 64)>(v, acc, tw, t);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) a.work[base + t + q * T] = v[q];
+        for (int q = 0; q < V; ++q) a.work[base + t + q * T] = v[q];
     } else if constexpr (MODE == AROW_ERR_REAL) {
 #pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = a.work[base + t + r * T];
+        for (int r = 0; r < V; ++r) v[r] = a.work[base + t + r * T];
         fft_line<N, -1, (G::T <= 64)>(v, acc, tw, t);
         double s[2] = {0.0, 0.0};  // error, norm
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < V; ++q) {
             const size_t i = base + t + q * T;
             const float dx = v[q].x * a.inv_nn, dy = v[q].y * a.inv_nn;
             const uint8_t sp = a.supp[i];
@@ -231,24 +231,24 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row_aux(AuxRowArgs a) {
     } else if constexpr (MODE == AROW_ABS_FWD_RHO) {
         const size_t sbase = (size_t)(a.src_index ? a.src_index[item] : item) * N * N + (size_t)y * N;
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
+        for (int r = 0; r < V; ++r) {
             const float2 d = a.src[sbase + t + r * T];
             v[r] = make_float2(sqrtf(d.x * d.x + d.y * d.y), 0.f);
         }
         fft_line<N, +1, (G::T <= 64)>(v, acc, tw, t);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) a.work[base + t + q * T] = v[q];
+        for (int q = 0; q < V; ++q) a.work[base + t + q * T] = v[q];
     } else if constexpr (MODE == AROW_CORR) {
         // two fields (work, work2) per item; first index of the strict maximum of |c|^2
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
             const float2* src = f == 0 ? a.work : a.work2;
 #pragma unroll
-            for (int r = 0; r < 8; ++r) v[r] = src[base + t + r * T];
+            for (int r = 0; r < V; ++r) v[r] = src[base + t + r * T];
             fft_line<N, -1, (G::T <= 64)>(v, acc, tw, t);
             unsigned long long best = 0;
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
+            for (int q = 0; q < V; ++q) {
                 const float val = v[q].x * v[q].x + v[q].y * v[q].y;
                 if (val > 0.f) {
                     const unsigned idx = (unsigned)(y * N + t + q * T);
@@ -291,7 +291,7 @@ struct AuxColArgs {
 template <int N, int MODE>
 __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_aux(AuxColArgs a) {
     using G = Geo<N>;
-    constexpr int T = G::T, C = G::C, NT = G::COL_THREADS;
+    constexpr int V = G::V, T = G::T, C = G::C, NT = G::COL_THREADS;
     __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[N * G::CS];
     __shared__ double red[4 * (NT / 64)];
@@ -305,26 +305,26 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_aux(AuxColArgs a) {
     const int x0 = (strip % TILES) * C;
     const size_t base = (size_t)item * N * N + x0 + c;
     const ColAcc<G::CS> acc{lds + c};
-    float2 v[8];
+    float2 v[V];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = a.f[base + (size_t)(t + r * T) * N];
+    for (int r = 0; r < V; ++r) v[r] = a.f[base + (size_t)(t + r * T) * N];
     __syncthreads();
     fft_line<N, +1>(v, acc, tw, t);
 
     if constexpr (MODE == ACOL_BLUR) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < V; ++q) {
             const float g = sw_gauss<N>(x0 + c, t + q * T, a.sigma_ft);
             v[q] = make_float2(v[q].x * g, v[q].y * g);
         }
         fft_line<N, -1>(v, acc, tw, t);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) a.f[base + (size_t)(t + q * T) * N] = v[q];
+        for (int q = 0; q < V; ++q) a.f[base + (size_t)(t + q * T) * N] = v[q];
     } else if constexpr (MODE == ACOL_ERR) {
         double s[2] = {0.0, 0.0};  // sum (A-|F|)^2, sum A^2 over A >= 0
         float fmax = 0.f;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < V; ++q) {
             const float A = a.amp[(size_t)(t + q * T) * N + x0 + c];
             const float m = sqrtf(v[q].y * v[q].y + v[q].x * v[q].x);
             fmax = vmax(fmax, m);
@@ -344,18 +344,18 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_aux(AuxColArgs a) {
         if (tid == 0) atomicMax(&a.maxbits[item], __float_as_uint(fmax));
         fft_line<N, -1>(v, acc, tw, t);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) a.f[base + (size_t)(t + q * T) * N] = v[q];
+        for (int q = 0; q < V; ++q) a.f[base + (size_t)(t + q * T) * N] = v[q];
     } else if constexpr (MODE == ACOL_FWD_UNIT) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < V; ++q) {
             const float m = sqrtf(v[q].x * v[q].x + v[q].y * v[q].y);
             const float s = m / m;
             a.f[base + (size_t)(t + q * T) * N] = make_float2(v[q].x * s, v[q].y * s);
         }
     } else if constexpr (MODE == ACOL_CORR) {
-        float2 d[8];
+        float2 d[V];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < V; ++q) {
             d[q] = v[q];
             if (a.unit_scale) {
                 const float m = sqrtf(d[q].x * d[q].x + d[q].y * d[q].y);
@@ -369,7 +369,7 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_aux(AuxColArgs a) {
         for (int sgn = 0; sgn < 2; ++sgn) {
             const float sg = sgn == 0 ? -1.f : 1.f;
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
+            for (int q = 0; q < V; ++q) {
                 const int ky = t + q * T;
                 float2 r;
                 if (a.ref_index) r = a.ref[(size_t)a.ref_index[item] * N * N + (size_t)ky * N + x0 + c];
@@ -388,7 +388,7 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_aux(AuxColArgs a) {
             fft_line<N, -1>(v, acc, tw, t);
             float2* dst = sgn == 0 ? a.f : a.f2;
 #pragma unroll
-            for (int q = 0; q < 8; ++q) dst[base + (size_t)(t + q * T) * N] = v[q];
+            for (int q = 0; q < V; ++q) dst[base + (size_t)(t + q * T) * N] = v[q];
         }
     }
 }
@@ -514,7 +514,7 @@ __device__ __forceinline__ size_t field_of(const FieldBatch& b, int item) {
 template <int N, int SIGN>
 __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_fft_rows(FieldBatch b) {
     using G = Geo<N>;
-    constexpr int T = G::T, NT = G::ROW_THREADS, WPF = N / G::RW;
+    constexpr int V = G::V, T = G::T, NT = G::ROW_THREADS, WPF = N / G::RW;
     __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[G::RW * G::RS];
     const int tid = threadIdx.x, t = tid % T, rl = tid / T;
@@ -525,20 +525,20 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_fft_rows(FieldBatch b) 
     float2* __restrict__ f = b.base + fo;
     const float2* __restrict__ s = b.src ? b.src + fo : f;
     const RowAcc acc{lds + rl * G::RS};
-    float2 v[8];
+    float2 v[V];
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = s[t + r * T];
+    for (int r = 0; r < V; ++r) v[r] = s[t + r * T];
     fft_line<N, SIGN, (G::T <= 64)>(v, acc, tw, t);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) f[t + q * T] = make_float2(v[q].x * b.scale, v[q].y * b.scale);
+    for (int q = 0; q < V; ++q) f[t + q * T] = make_float2(v[q].x * b.scale, v[q].y * b.scale);
 }
 
 // Column transforms (all columns of each listed field), in place.
 template <int N, int SIGN>
 __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_fft_cols(FieldBatch b) {
     using G = Geo<N>;
-    constexpr int T = G::T, C = G::C, NT = G::COL_THREADS, TILES = N / C;
+    constexpr int V = G::V, T = G::T, C = G::C, NT = G::COL_THREADS, TILES = N / C;
     __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[N * G::CS];
     const int tid = threadIdx.x, c = tid % C, t = tid / C;
@@ -549,13 +549,13 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_fft_cols(FieldBatch b) 
     float2* __restrict__ f = b.base + field_of(b, item) * N * N + x0;
     const int off = t * N + c;
     const ColAcc<G::CS> acc{lds + c};
-    float2 v[8];
+    float2 v[V];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = f[off + r * T * N];
+    for (int r = 0; r < V; ++r) v[r] = f[off + r * T * N];
     __syncthreads();
     fft_line<N, SIGN>(v, acc, tw, t);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) f[off + q * T * N] = make_float2(v[q].x * b.scale, v[q].y * b.scale);
+    for (int q = 0; q < V; ++q) f[off + q * T * N] = make_float2(v[q].x * b.scale, v[q].y * b.scale);
 }
 
 // ---------------------------------------------------------------------------
@@ -583,7 +583,7 @@ __device__ __forceinline__ float2 unit_scaled(float2 z) {
 template <int N>
 __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_corr_spec_cols(CorrSpecArgs a) {
     using G = Geo<N>;
-    constexpr int T = G::T, C = G::C, NT = G::COL_THREADS, TILES = N / C;
+    constexpr int V = G::V, T = G::T, C = G::C, NT = G::COL_THREADS, TILES = N / C;
     __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[N * G::CS];
     const int tid = threadIdx.x, c = tid % C, t = tid / C;
@@ -600,9 +600,9 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_corr_spec_cols(CorrSpec
 #pragma unroll
     for (int sgn = 0; sgn < 2; ++sgn) {
         const float sg = sgn == 0 ? -1.f : 1.f;
-        float2 v[8];
+        float2 v[V];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < V; ++q) {
             const int ky = t + q * T;
             float2 dv = d[off + q * T * N], rv = r[off + q * T * N];
             if (a.unit_scale) {
@@ -623,7 +623,7 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_corr_spec_cols(CorrSpec
         fft_line<N, -1>(v, acc, tw, t);
         float2* __restrict__ o = (sgn == 0 ? a.out1 : a.out2) + (size_t)item * N * N + x0;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) o[off + q * T * N] = v[q];
+        for (int q = 0; q < V; ++q) o[off + q * T * N] = v[q];
     }
 }
 

@@ -170,15 +170,20 @@ template <int N> int launch_col(spr_ctx* c, int mode, float2* f, int cnt, hipStr
     return SPR_OK;
 }
 
-#define SPR_DISPATCH_N(c, FN, ...)                         \
-    switch ((c)->n) {                                      \
-        case 64: return FN<64>(__VA_ARGS__);               \
-        case 128: return FN<128>(__VA_ARGS__);             \
-        case 256: return FN<256>(__VA_ARGS__);             \
-        case 512: return FN<512>(__VA_ARGS__);             \
-        case 1024: return FN<1024>(__VA_ARGS__);           \
-        case 2048: return FN<2048>(__VA_ARGS__);           \
-        default: return fail((c), SPR_ERR_ARG, "unsupported grid size"); \
+// Grid sizes with compiled kernels (RadixPlan in spr_fft.hpp).
+#define SPR_SIZES(X) X(64) X(128) X(256) X(384) X(512) X(768) X(1024) X(2048)
+
+#define SPR_DISPATCH_N(c, FN, ...)                                          \
+    switch ((c)->n) {                                                       \
+        case 64: return FN<64>(__VA_ARGS__);                                \
+        case 128: return FN<128>(__VA_ARGS__);                              \
+        case 256: return FN<256>(__VA_ARGS__);                              \
+        case 384: return FN<384>(__VA_ARGS__);                              \
+        case 512: return FN<512>(__VA_ARGS__);                              \
+        case 768: return FN<768>(__VA_ARGS__);                              \
+        case 1024: return FN<1024>(__VA_ARGS__);                            \
+        case 2048: return FN<2048>(__VA_ARGS__);                            \
+        default: return fail((c), SPR_ERR_ARG, "unsupported grid size");   \
     }
 
 int row_n(spr_ctx* c, int mode, const RowArgs& a, int cnt, hipStream_t s) {
@@ -262,8 +267,19 @@ int col_aux(spr_ctx* c, int mode, const AuxColArgs& a, int items) {
     SPR_DISPATCH_N(c, launch_col_aux, c, mode, a, items);
 }
 
-int rows_per_wg(int n) { return 256 / (n / 8); }
-int cols_per_wg(int n) { return (8192 / n) < n ? (8192 / n) : n; }
+// Rows per row-pass workgroup and columns per column-pass workgroup of size n.
+int rows_per_wg(int n) {
+#define SPR_RW_CASE(N_) if (n == N_) return Geo<N_>::RW;
+    SPR_SIZES(SPR_RW_CASE)
+#undef SPR_RW_CASE
+    return 1;
+}
+int cols_per_wg(int n) {
+#define SPR_C_CASE(N_) if (n == N_) return Geo<N_>::C;
+    SPR_SIZES(SPR_C_CASE)
+#undef SPR_C_CASE
+    return 1;
+}
 
 AuxRowArgs aux_row_args(spr_ctx* c) {
     AuxRowArgs a{};
@@ -689,7 +705,10 @@ int check_indices(spr_ctx* c, const int32_t* idx, int count, int limit, const ch
 extern "C" {
 
 int spr_supported_size(int n) {
-    return (n == 64 || n == 128 || n == 256 || n == 512 || n == 1024 || n == 2048) ? 1 : 0;
+#define SPR_SUPPORTED_CASE(N_) if (n == N_) return 1;
+    SPR_SIZES(SPR_SUPPORTED_CASE)
+#undef SPR_SUPPORTED_CASE
+    return 0;
 }
 
 int spr_create(spr_ctx** out, int n, int capacity, const float* amp, int device) {

@@ -1,22 +1,25 @@
 // Device-side FFT building blocks for gfx950 (CDNA4).
 //
-// One "line" (a row or a column of an N x N field) is transformed by T = N/8
-// threads that each hold 8 complex values in registers. The transform is a
-// Stockham autosort FFT: stage s (radix R, span Ns = product of the earlier
-// radices) takes x[b + r*N/R] for butterfly b, multiplies input r by
-// w^(r*(b mod Ns)) with w = exp(SIGN*2*pi*i/(Ns*R)), applies an R-point DFT and
-// writes y[(b/Ns)*Ns*R + (b mod Ns) + q*Ns]. Stages exchange data through LDS;
-// the first and the last stage are radix 8 so that thread t holds
-// x[t + r*T] on entry and X[t + q*T] on exit. A forward and an inverse transform
-// can therefore be chained in registers, with a per-element operation between
-// them, without touching LDS.
+// One "line" (a row or a column of an N x N field) is transformed by
+// T = N/V threads that each hold V complex values in registers (V = 8 for
+// power-of-two N, V = 12 for N = 384 and 768). The transform is a Stockham
+// autosort FFT: stage s (radix R, span Ns = product of the earlier radices)
+// takes x[b + r*N/R] for butterfly b, multiplies input r by w^(r*(b mod Ns))
+// with w = exp(SIGN*2*pi*i/(Ns*R)), applies an R-point DFT and writes
+// y[(b/Ns)*Ns*R + (b mod Ns) + q*Ns]. Thread t owns the butterflies t + T*u,
+// u < V/R, of every stage. Stages exchange data through LDS; thread t holds
+// x[t + j*T] (j < V) on entry and X[t + j*T] on exit (the first and the last
+// stage see a compile-time permutation of these registers when their radix
+// is below V). A forward and an inverse transform can therefore be chained in
+// registers, with a per-element operation between them, without touching LDS.
 //
 // Twiddles come from a per-workgroup LDS table laid out per stage
 // (TwTable<N>): for stage s with span Ns and radix R, row k in [0, Ns) holds
-// w^(r*k), r = 1..R-1, densely at the odd stride R-1 so that the 64 lanes of a
-// wave, which read consecutive k, spread over the LDS banks. The table is built from
-// root[m] = exp(-2*pi*i*m/N), computed in double on the host and rounded to
-// float, so every twiddle equals the float rounding of the exact root.
+// w^(r*k), r = 1..R-1, densely at the stride R-1 (odd for every radix used),
+// so that the 64 lanes of a wave, which read consecutive k, spread over the LDS
+// banks. The table is built from root[m] = exp(-2*pi*i*m/N), computed in double
+// on the host and rounded to float, so every twiddle equals the float rounding
+// of the exact root.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -69,21 +72,97 @@ template <int S> __device__ __forceinline__ void dft8(float2* v) {
     v[7] = csub(e3, p3);
 }
 
+constexpr float kH3 = 0.86602540378443864676f;   // sqrt(3)/2
+
+// X[k] = sum_n a[n] exp(S*2*pi*i*n*k/3)
+template <int S> __device__ __forceinline__ void dft3(float2& a0, float2& a1, float2& a2) {
+    const float2 t1 = cadd(a1, a2), t2 = csub(a1, a2);
+    const float2 m1 = make_float2(a0.x - 0.5f * t1.x, a0.y - 0.5f * t1.y);
+    const float2 m2 = mul_si<S>(make_float2(kH3 * t2.x, kH3 * t2.y));
+    a0 = cadd(a0, t1);
+    a1 = cadd(m1, m2);
+    a2 = csub(m1, m2);
+}
+
+// z * exp(S*2*pi*i*M/12), M a compile-time multiple of 30 degrees.
+template <int M, int S> __device__ __forceinline__ float2 rot12(float2 z) {
+    constexpr int m = M % 12;
+    if constexpr (m == 0) {
+        return z;
+    } else if constexpr (m == 3) {
+        return mul_si<S>(z);
+    } else if constexpr (m == 6) {
+        return make_float2(-z.x, -z.y);
+    } else {
+        static_assert(m == 1 || m == 2 || m == 4, "twiddle");
+        // cos and sin of m*30 degrees
+        constexpr float c = m == 1 ? kH3 : m == 2 ? 0.5f : -0.5f;
+        constexpr float s = m == 1 ? 0.5f : kH3;
+        return cmul(z, make_float2(c, S * s));
+    }
+}
+
+// 6-point DFT as 2 x 3 (Cooley-Tukey, n = n1 + 2*n2, k = k2 + 3*k1).
+template <int S> __device__ __forceinline__ void dft6(float2* v) {
+    float2 a0 = v[0], a1 = v[2], a2 = v[4];
+    float2 b0 = v[1], b1 = v[3], b2 = v[5];
+    dft3<S>(a0, a1, a2);
+    dft3<S>(b0, b1, b2);
+    b1 = rot12<2, S>(b1);
+    b2 = rot12<4, S>(b2);
+    v[0] = cadd(a0, b0);
+    v[3] = csub(a0, b0);
+    v[1] = cadd(a1, b1);
+    v[4] = csub(a1, b1);
+    v[2] = cadd(a2, b2);
+    v[5] = csub(a2, b2);
+}
+
+// 12-point DFT as 4 x 3 (Cooley-Tukey, n = n1 + 4*n2, k = k2 + 3*k1): four
+// 3-point DFTs, the twiddles w12^(n1*k2), three 4-point DFTs.
+template <int S> __device__ __forceinline__ void dft12(float2* v) {
+    float2 y0[3] = {v[0], v[4], v[8]}, y1[3] = {v[1], v[5], v[9]};
+    float2 y2[3] = {v[2], v[6], v[10]}, y3[3] = {v[3], v[7], v[11]};
+    dft3<S>(y0[0], y0[1], y0[2]);
+    dft3<S>(y1[0], y1[1], y1[2]);
+    dft3<S>(y2[0], y2[1], y2[2]);
+    dft3<S>(y3[0], y3[1], y3[2]);
+    y1[1] = rot12<1, S>(y1[1]);
+    y1[2] = rot12<2, S>(y1[2]);
+    y2[1] = rot12<2, S>(y2[1]);
+    y2[2] = rot12<4, S>(y2[2]);
+    y3[1] = rot12<3, S>(y3[1]);
+    y3[2] = rot12<6, S>(y3[2]);
+#pragma unroll
+    for (int k2 = 0; k2 < 3; ++k2) {
+        float2 a0 = y0[k2], a1 = y1[k2], a2 = y2[k2], a3 = y3[k2];
+        dft4<S>(a0, a1, a2, a3);
+        v[k2] = a0;
+        v[k2 + 3] = a1;
+        v[k2 + 6] = a2;
+        v[k2 + 9] = a3;
+    }
+}
+
 template <int R, int S> __device__ __forceinline__ void dftR(float2* v) {
-    if constexpr (R == 8) dft8<S>(v);
+    if constexpr (R == 12) dft12<S>(v);
+    else if constexpr (R == 8) dft8<S>(v);
+    else if constexpr (R == 6) dft6<S>(v);
     else if constexpr (R == 4) dft4<S>(v[0], v[1], v[2], v[3]);
+    else if constexpr (R == 3) dft3<S>(v[0], v[1], v[2]);
     else dft2<S>(v[0], v[1]);
 }
 
-// Radix plan: N = 8 * middle * 8 with middle in {1, 2, 4, 8, 16(=2x8), ...}.
-// Stage radices (first and last are 8).
+// Radix plans: V values per thread, ns stages with radices r[] (each dividing V).
 template <int N> struct RadixPlan;
-template <> struct RadixPlan<64>   { static constexpr int ns = 2; static constexpr int r[4] = {8, 8, 0, 0}; };
-template <> struct RadixPlan<128>  { static constexpr int ns = 3; static constexpr int r[4] = {8, 2, 8, 0}; };
-template <> struct RadixPlan<256>  { static constexpr int ns = 3; static constexpr int r[4] = {8, 4, 8, 0}; };
-template <> struct RadixPlan<512>  { static constexpr int ns = 3; static constexpr int r[4] = {8, 8, 8, 0}; };
-template <> struct RadixPlan<1024> { static constexpr int ns = 4; static constexpr int r[4] = {8, 2, 8, 8}; };
-template <> struct RadixPlan<2048> { static constexpr int ns = 4; static constexpr int r[4] = {8, 4, 8, 8}; };
+template <> struct RadixPlan<64>   { static constexpr int V = 8;  static constexpr int ns = 2; static constexpr int r[4] = {8, 8, 0, 0}; };
+template <> struct RadixPlan<128>  { static constexpr int V = 8;  static constexpr int ns = 3; static constexpr int r[4] = {8, 2, 8, 0}; };
+template <> struct RadixPlan<256>  { static constexpr int V = 8;  static constexpr int ns = 3; static constexpr int r[4] = {8, 4, 8, 0}; };
+template <> struct RadixPlan<384>  { static constexpr int V = 12; static constexpr int ns = 4; static constexpr int r[4] = {12, 4, 4, 2}; };
+template <> struct RadixPlan<512>  { static constexpr int V = 8;  static constexpr int ns = 3; static constexpr int r[4] = {8, 8, 8, 0}; };
+template <> struct RadixPlan<768>  { static constexpr int V = 12; static constexpr int ns = 4; static constexpr int r[4] = {12, 4, 4, 4}; };
+template <> struct RadixPlan<1024> { static constexpr int V = 8;  static constexpr int ns = 4; static constexpr int r[4] = {8, 2, 8, 8}; };
+template <> struct RadixPlan<2048> { static constexpr int V = 8;  static constexpr int ns = 4; static constexpr int r[4] = {8, 4, 8, 8}; };
 
 template <int N> constexpr int span_before(int s) {
     int ns = 1;
@@ -93,9 +172,9 @@ template <int N> constexpr int span_before(int s) {
 
 // Per-stage twiddle tables in LDS. Stage s >= 1 (radix R, span NS) needs
 // w^(r*k) = root[r*k*N/(NS*R)] for k < NS, 1 <= r < R; they are stored densely
-// at tws[off(s) + k*(R-1) + r-1]. R is a power of two, so the row stride R-1 is
-// odd and the 8-byte reads of 16 consecutive k fall in distinct LDS bank pairs;
-// one base address serves all r.
+// at tws[off(s) + k*(R-1) + r-1]. The row stride R-1 is odd (R in {2, 4, 8, 12}),
+// so the 8-byte reads of 16 consecutive k fall in distinct LDS bank pairs; one
+// base address serves all r.
 template <int N> struct TwTable {
     static constexpr int off(int s) {
         int o = 0;
@@ -127,18 +206,23 @@ __device__ __forceinline__ void tw_fill(float2* tws, const float2* __restrict__ 
     tw_fill_stage<N, 1>(tws, root, tid, nt);
 }
 
-// Twiddle and butterfly of stage STAGE on registers v (8 values = 8/R butterflies).
+// Values per thread and threads per line of the N-point transform.
+template <int N> constexpr int line_values() { return RadixPlan<N>::V; }
+template <int N> constexpr int line_threads() { return N / RadixPlan<N>::V; }
+
+// Twiddle and butterfly of stage STAGE on registers v (V values = V/R butterflies).
 template <int N, int STAGE, int SIGN>
-__device__ __forceinline__ void stage_compute(float2 (&v)[8], const float2* __restrict__ tws, int t) {
+__device__ __forceinline__ void stage_compute(float2 (&v)[RadixPlan<N>::V], const float2* __restrict__ tws, int t) {
     constexpr int R = RadixPlan<N>::r[STAGE];
     constexpr int NS = span_before<N>(STAGE);
-    constexpr int T = N / 8;
-    constexpr int NB = 8 / R;
+    constexpr int T = line_threads<N>();
+    constexpr int NB = RadixPlan<N>::V / R;
+    static_assert(NB * R == RadixPlan<N>::V, "radix must divide the values per thread");
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
         if constexpr (NS > 1) {
             const int b = t + T * u;
-            const int k = b & (NS - 1);
+            const int k = b % NS;
             const float2* w = tws + TwTable<N>::off(STAGE) + k * (R - 1);
 #pragma unroll
             for (int r = 1; r < R; ++r) {
@@ -153,10 +237,10 @@ __device__ __forceinline__ void stage_compute(float2 (&v)[8], const float2* __re
 
 // Stage inputs of stage STAGE from LDS: v[u*R + r] = x[b + r*N/R], b = t + T*u.
 template <int N, int STAGE, class Acc>
-__device__ __forceinline__ void stage_load(float2 (&v)[8], const Acc& acc, int t) {
+__device__ __forceinline__ void stage_load(float2 (&v)[RadixPlan<N>::V], const Acc& acc, int t) {
     constexpr int R = RadixPlan<N>::r[STAGE];
-    constexpr int T = N / 8;
-    constexpr int NB = 8 / R;
+    constexpr int T = line_threads<N>();
+    constexpr int NB = RadixPlan<N>::V / R;
     constexpr int M = N / R;
 #pragma unroll
     for (int u = 0; u < NB; ++u)
@@ -166,17 +250,46 @@ __device__ __forceinline__ void stage_load(float2 (&v)[8], const Acc& acc, int t
 
 // Stage outputs of stage STAGE to LDS: y[(b/Ns)*Ns*R + b%Ns + q*Ns] = v[u*R + q].
 template <int N, int STAGE, class Acc>
-__device__ __forceinline__ void stage_store(const float2 (&v)[8], const Acc& acc, int t) {
+__device__ __forceinline__ void stage_store(const float2 (&v)[RadixPlan<N>::V], const Acc& acc, int t) {
     constexpr int R = RadixPlan<N>::r[STAGE];
     constexpr int NS = span_before<N>(STAGE);
-    constexpr int T = N / 8;
-    constexpr int NB = 8 / R;
+    constexpr int T = line_threads<N>();
+    constexpr int NB = RadixPlan<N>::V / R;
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
         const int b = t + T * u;
-        const int base = (b / NS) * NS * R + (b & (NS - 1));
+        const int base = (b / NS) * NS * R + b % NS;
 #pragma unroll
         for (int q = 0; q < R; ++q) acc.st(base + q * NS, v[u * R + q]);
+    }
+}
+
+// Register order of the first stage (radix R0, NB0 = V/R0 butterflies): it reads
+// x[t + T*(u + r*NB0)] as v[u*R0 + r], while the caller supplies x[t + T*j] as
+// v[j]. The last stage (radix RL, NBL = V/RL) leaves X[t + T*(u + q*NBL)] in
+// v[u*RL + q]. Both reorderings are compile-time register renames.
+template <int V, int R> __device__ __forceinline__ void regs_to_stage(float2 (&v)[V]) {
+    if constexpr (R < V) {
+        constexpr int NB = V / R;
+        float2 w[V];
+#pragma unroll
+        for (int u = 0; u < NB; ++u)
+#pragma unroll
+            for (int r = 0; r < R; ++r) w[u * R + r] = v[u + r * NB];
+#pragma unroll
+        for (int j = 0; j < V; ++j) v[j] = w[j];
+    }
+}
+template <int V, int R> __device__ __forceinline__ void stage_to_regs(float2 (&v)[V]) {
+    if constexpr (R < V) {
+        constexpr int NB = V / R;
+        float2 w[V];
+#pragma unroll
+        for (int u = 0; u < NB; ++u)
+#pragma unroll
+            for (int q = 0; q < R; ++q) w[u + q * NB] = v[u * R + q];
+#pragma unroll
+        for (int j = 0; j < V; ++j) v[j] = w[j];
     }
 }
 
@@ -194,14 +307,17 @@ template <bool WAVE> __device__ __forceinline__ void line_sync() {
     }
 }
 
-// Full line transform. On entry v[r] = x[t + r*N/8]; on exit v[q] = X[t + q*N/8]
-// with X[k] = sum_j x[j] exp(SIGN*2*pi*i*j*k/N) (unnormalised). With WAVE = false
-// every thread of the workgroup must call it (it contains workgroup barriers);
-// with WAVE = true the N/8 threads of each line must lie in one wavefront and
-// the line's LDS must be touched by that wavefront only.
+// Full line transform. On entry v[j] = x[t + j*T]; on exit v[j] = X[t + j*T]
+// (T = N/V) with X[k] = sum_n x[n] exp(SIGN*2*pi*i*n*k/N) (unnormalised). With
+// WAVE = false every thread of the workgroup must call it (it contains
+// workgroup barriers); with WAVE = true the T threads of each line must lie in
+// one wavefront and the line's LDS must be touched by that wavefront only.
 template <int N, int SIGN, bool WAVE = false, class Acc>
-__device__ __forceinline__ void fft_line(float2 (&v)[8], const Acc& acc, const float2* __restrict__ tw, int t) {
+__device__ __forceinline__ void fft_line(float2 (&v)[RadixPlan<N>::V], const Acc& acc, const float2* __restrict__ tw,
+                                         int t) {
     constexpr int NSTAGE = RadixPlan<N>::ns;
+    constexpr int V = RadixPlan<N>::V;
+    regs_to_stage<V, RadixPlan<N>::r[0]>(v);
     stage_compute<N, 0, SIGN>(v, tw, t);
     stage_store<N, 0>(v, acc, t);
     line_sync<WAVE>();
@@ -221,6 +337,7 @@ __device__ __forceinline__ void fft_line(float2 (&v)[8], const Acc& acc, const f
         stage_load<N, 3>(v, acc, t);
         stage_compute<N, 3, SIGN>(v, tw, t);
     }
+    stage_to_regs<V, RadixPlan<N>::r[NSTAGE - 1]>(v);
     // Leave LDS free for the next user of this buffer.
     line_sync<WAVE>();
 }

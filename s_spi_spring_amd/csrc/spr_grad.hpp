@@ -29,7 +29,7 @@ struct LossArgs {
 template <int N>
 __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_loss(LossArgs a) {
     using G = Geo<N>;
-    constexpr int T = G::T, C = G::C, NT = G::COL_THREADS, TILES = N / C;
+    constexpr int V = G::V, T = G::T, C = G::C, NT = G::COL_THREADS, TILES = N / C;
     __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[N * G::CS];
     __shared__ double red[4 * (NT / 64)];
@@ -42,10 +42,10 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_loss(LossArgs a) {
     const float* __restrict__ amp = a.amp + x0;
     const int off = t * N + c;
     const ColAcc<G::CS> acc{lds + c};
-    float2 v[8];
-    float Aq[8];
+    float2 v[V];
+    float Aq[V];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
+    for (int r = 0; r < V; ++r) {
         v[r] = f[off + r * T * N];
         Aq[r] = amp[off + r * T * N];
     }
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_loss(LossArgs a) {
     fft_line<N, +1>(v, acc, tw, t);
     double l[1] = {0.0};
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < V; ++q) {
         const float A = Aq[q];
         const float u = v[q].x * v[q].x + v[q].y * v[q].y;
         float w = 0.f;
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_loss(LossArgs a) {
     if (tid == 0) a.partial[strip] = l[0];
     fft_line<N, -1>(v, acc, tw, t);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) f[off + q * T * N] = v[q];
+    for (int q = 0; q < V; ++q) f[off + q * T * N] = v[q];
 }
 
 // Per-individual step size of the support-projected refinement:
@@ -127,7 +127,7 @@ struct AdamRowArgs {
 template <int N, bool MASKED>
 __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row_adam(AdamRowArgs a) {
     using G = Geo<N>;
-    constexpr int T = G::T;
+    constexpr int V = G::V, T = G::T;
     constexpr bool WV = T <= 64;
     __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[G::RW * G::RS];
@@ -144,13 +144,13 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row_adam(AdamRowArgs a)
     float2* __restrict__ work = a.work + blk;
     const RowAcc acc{lds + rl * G::RS};
     const float lr = MASKED ? a.lrp[field] : a.lr;
-    float2 v[8];
+    float2 v[V];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = work[off + r * T];
+    for (int r = 0; r < V; ++r) v[r] = work[off + r * T];
     __syncthreads();
     fft_line<N, -1, WV>(v, acc, tw, t);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < V; ++q) {
         const int i = off + q * T;
         const float2 gi = make_float2(v[q].x * 2.f, v[q].y * 2.f);
         float2 xi = x[i];
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row_adam(AdamRowArgs a)
     }
     fft_line<N, +1, WV>(v, acc, tw, t);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) work[off + q * T] = v[q];
+    for (int q = 0; q < V; ++q) work[off + q * T] = v[q];
 }
 
 }  // namespace spr

@@ -26,8 +26,11 @@ namespace spr {
 // ---------------------------------------------------------------------------
 // Launch geometry
 // ---------------------------------------------------------------------------
+constexpr int pow2_floor(int x) { return x >= 2 ? 2 * pow2_floor(x / 2) : 1; }
+
 template <int N> struct Geo {
-    static constexpr int T = N / 8;                       // threads per line
+    static constexpr int V = RadixPlan<N>::V;             // values per thread
+    static constexpr int T = N / V;                       // threads per line
     static constexpr int ROW_THREADS = 256;
     static constexpr int RW = ROW_THREADS / T;            // rows per row-pass workgroup
     static constexpr int RS = N + N / 8;                  // padded LDS row length (float2)
@@ -35,7 +38,7 @@ template <int N> struct Geo {
 #define SPR_COL_COLUMNS 8192   // columns x rows of a column-pass tile
 #endif
 #ifndef SPR_COL_WAVE
-#define SPR_COL_WAVE 0         // 1: wavefront-owned columns when N/8 <= 64
+#define SPR_COL_WAVE 0         // 1: wavefront-owned columns when N/V <= 64
 #endif
 #ifndef SPR_COL_XCD
 #define SPR_COL_XCD 1          // XCD-aware strip order for strips narrower than 128 bytes
@@ -43,7 +46,8 @@ template <int N> struct Geo {
 #ifndef SPR_COL_AMP_PREFETCH
 #define SPR_COL_AMP_PREFETCH 1 // load the amplitude with the field, before the forward transform
 #endif
-    static constexpr int C = (SPR_COL_COLUMNS / N) < N ? (SPR_COL_COLUMNS / N) : N;  // columns per column-pass workgroup
+    // columns per column-pass workgroup: a power of two (it divides N)
+    static constexpr int C = pow2_floor(SPR_COL_COLUMNS / N) < N ? pow2_floor(SPR_COL_COLUMNS / N) : N;
     static constexpr bool COL_WAVE = SPR_COL_WAVE && T <= 64;
     static constexpr int COL_THREADS = C * T;
     static constexpr int CS = C + 1;                      // padded LDS row length of a column tile
@@ -71,8 +75,8 @@ template <int N> struct Geo {
     static constexpr int COL_WG = COL_WG_LDS < COL_WG_SLOTS ? (COL_WG_LDS < SPR_COL_WG_PER_CU ? COL_WG_LDS : SPR_COL_WG_PER_CU)
                                                             : (COL_WG_SLOTS < SPR_COL_WG_PER_CU ? COL_WG_SLOTS : SPR_COL_WG_PER_CU);
     static constexpr int COL_WAVES_PER_EU = (COL_WG > 0 ? COL_WG : 1) * COL_THREADS / 256;
-    static_assert(RW >= 1 && ROW_THREADS % T == 0, "row geometry");
-    static_assert(COL_THREADS <= 1024, "column geometry");
+    static_assert(RW >= 1 && ROW_THREADS % T == 0 && N % RW == 0, "row geometry");
+    static_assert(COL_THREADS <= 1024 && N % C == 0, "column geometry");
 };
 
 // Loads and stores of the streamed fields (each value is read and written once
@@ -202,7 +206,7 @@ struct RowArgs {
     const uint8_t* supp;
     const float2* tw;
     float beta;
-    float inv_nn;  // 1/N^2 (exact for power-of-two N)
+    float inv_nn;  // 1/N^2 (exact for power-of-two N; within 1 ulp of dividing by N^2 otherwise)
     PhaseCut pc;
 };
 
@@ -217,7 +221,7 @@ struct RowArgs {
 template <int N, int MODE>
 __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
     using G = Geo<N>;
-    constexpr int T = G::T;
+    constexpr int V = G::V, T = G::T;
     constexpr bool WV = T <= 64;   // every row inside one wavefront: no workgroup barriers
     __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[G::RW * G::RS];
@@ -232,37 +236,37 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
     const uint8_t* __restrict__ supp = a.supp + blk;
     const int off = rl * N + t;
     const RowAcc acc{lds + rl * G::RS};
-    float2 v[8];
+    float2 v[V];
     __syncthreads();
 
     if constexpr (MODE == ROW_FWD || MODE == ROW_FWD_INPLACE) {
 #pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = rho[off + r * T];
+        for (int r = 0; r < V; ++r) v[r] = rho[off + r * T];
         fft_line<N, +1, WV>(v, acc, tw, t);
         float2* dst = (MODE == ROW_FWD) ? work : rho;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) dst[off + q * T] = v[q];
+        for (int q = 0; q < V; ++q) dst[off + q * T] = v[q];
     } else if constexpr (MODE == ROW_FINAL || MODE == ROW_INV_INPLACE) {
         const float2* src = (MODE == ROW_FINAL) ? work : rho;
 #pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = src[off + r * T];
+        for (int r = 0; r < V; ++r) v[r] = src[off + r * T];
         fft_line<N, -1, WV>(v, acc, tw, t);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) rho[off + q * T] = make_float2(v[q].x * a.inv_nn, v[q].y * a.inv_nn);
+        for (int q = 0; q < V; ++q) rho[off + q * T] = make_float2(v[q].x * a.inv_nn, v[q].y * a.inv_nn);
     } else {
         // The state and the support may be loaded with the field, before the
         // transform, so that their latency hides behind it.
-        float2 bn[8];
-        uint8_t sn[8];
+        float2 bn[V];
+        uint8_t sn[V];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) {
+        for (int r = 0; r < V; ++r) {
             v[r] = ld_stream(work + off + r * T);
             if constexpr (SPR_ROW_PREFETCH >= 2) sn[r] = supp[off + r * T];
             if constexpr (MODE != ROW_ER && SPR_ROW_PREFETCH >= 1) bn[r] = ld_stream(rho + off + r * T);
         }
         fft_line<N, -1, WV>(v, acc, tw, t);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int q = 0; q < V; ++q) {
             const int i = off + q * T;
             const float2 d = make_float2(v[q].x * a.inv_nn, v[q].y * a.inv_nn);  // rho'
             const uint8_t s = SPR_ROW_PREFETCH >= 2 ? sn[q] : supp[i];
@@ -289,7 +293,7 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
         }
         fft_line<N, +1, WV>(v, acc, tw, t);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) st_stream(work + off + q * T, v[q]);
+        for (int q = 0; q < V; ++q) st_stream(work + off + q * T, v[q]);
     }
 }
 
@@ -321,6 +325,7 @@ template <int N> __device__ __forceinline__ int col_strip(int b) {
         return b;
     } else {
         constexpr int CH = 8 * GS;              // workgroups per chunk of lines
+        static_assert((N / C) % CH == 0, "a field's strips must fill whole chunks (the remap stays in range)");
         const int j = b % CH;
         return (b - j) + (j % 8) * GS + j / 8;
     }
@@ -337,7 +342,7 @@ template <int N, int MODE>
 __global__ __launch_bounds__(Geo<N>::COL_THREADS) __attribute__((amdgpu_waves_per_eu(Geo<N>::COL_WAVES_PER_EU)))
 void k_col(ColArgs a) {
     using G = Geo<N>;
-    constexpr int T = G::T, C = G::C, CS = G::CS;
+    constexpr int V = G::V, T = G::T, C = G::C, CS = G::CS;
     constexpr bool WV = G::COL_WAVE;
     __shared__ float2 tw[TwTable<N>::SIZE];
     __shared__ float2 lds[G::COL_TILE];
@@ -352,51 +357,51 @@ void k_col(ColArgs a) {
     // Block-uniform bases with 32-bit per-thread offsets (SGPR base addressing).
     float2* __restrict__ f = a.f + (size_t)p * N * N + x0;
     const int off = t * N + c;
-    float2 v[8];
+    float2 v[V];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) v[r] = ld_stream(f + off + r * T * N);
+    for (int r = 0; r < V; ++r) v[r] = ld_stream(f + off + r * T * N);
     if constexpr (WV) {
         constexpr int CST = G::CST;
         const int cc = tid / T;   // column owned by this thread's wavefront
         const int tl = tid % T;
         const RowAcc mine{lds + c * CST};
 #pragma unroll
-        for (int r = 0; r < 8; ++r) mine.stk<T>(t, r, v[r]);
+        for (int r = 0; r < V; ++r) mine.stk<T>(t, r, v[r]);
         __syncthreads();
         const RowAcc acc{lds + cc * CST};
 #pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = acc.ldk<T>(tl, r);
+        for (int r = 0; r < V; ++r) v[r] = acc.ldk<T>(tl, r);
         if constexpr (MODE == COL_PROJ || MODE == COL_FWD) fft_line<N, +1, true>(v, acc, tw, tl);
         if constexpr (MODE == COL_PROJ) {
             const float* __restrict__ at = a.amp_t + (x0 + cc) * N;
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = project_modulus(v[q], at[tl + q * T]);
+            for (int q = 0; q < V; ++q) v[q] = project_modulus(v[q], at[tl + q * T]);
         }
         if constexpr (MODE == COL_PROJ || MODE == COL_INV) fft_line<N, -1, true>(v, acc, tw, tl);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) acc.stk<T>(tl, q, v[q]);
+        for (int q = 0; q < V; ++q) acc.stk<T>(tl, q, v[q]);
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < 8; ++r) v[r] = mine.ldk<T>(t, r);
+        for (int r = 0; r < V; ++r) v[r] = mine.ldk<T>(t, r);
     } else {
         const float* __restrict__ amp = a.amp + x0;
         const ColAcc<CS> acc{lds + c};
-        float am[8];
+        float am[V];
         if constexpr (MODE == COL_PROJ && SPR_COL_AMP_PREFETCH) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) am[q] = amp[off + q * T * N];
+            for (int q = 0; q < V; ++q) am[q] = amp[off + q * T * N];
         }
         __syncthreads();
         if constexpr (MODE == COL_PROJ || MODE == COL_FWD) fft_line<N, +1>(v, acc, tw, t);
         if constexpr (MODE == COL_PROJ) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
+            for (int q = 0; q < V; ++q)
                 v[q] = project_modulus(v[q], SPR_COL_AMP_PREFETCH ? am[q] : amp[off + q * T * N]);
         }
         if constexpr (MODE == COL_PROJ || MODE == COL_INV) fft_line<N, -1>(v, acc, tw, t);
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) st_stream(f + off + q * T * N, v[q]);
+    for (int q = 0; q < V; ++q) st_stream(f + off + q * T * N, v[q]);
 }
 
 }  // namespace spr

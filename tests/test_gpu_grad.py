@@ -36,7 +36,7 @@ def np_loss_grad(rho, amp, kind, eps):
 
 
 @pytest.mark.parametrize("kind", [0, 1])
-@pytest.mark.parametrize("n", [128, 512])
+@pytest.mark.parametrize("n", [128, 384, 512])
 def test_gradient_vs_fp64_and_fd(hip, n, kind):
     import torch
     amp, rho, supp = case(n, 2, seed=21)

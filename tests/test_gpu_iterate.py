@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 
-@pytest.mark.parametrize("n", [64, 128, 256, 512, 1024])
+@pytest.mark.parametrize("n", [64, 128, 256, 384, 512, 768, 1024])
 def test_ft_ift(hip, orc, n):
     amp, rho, supp = case(n, 2)
     be = hip.Backend(amp, capacity=2)
@@ -30,7 +30,7 @@ def test_ft_ift(hip, orc, n):
     assert rel_l2(back, rho) < 1e-6
 
 
-@pytest.mark.parametrize("n,P", [(128, 8), (512, 4)])
+@pytest.mark.parametrize("n,P", [(128, 8), (384, 3), (512, 4), (768, 2)])
 @pytest.mark.parametrize("alg,beta", [("HIO", 0.9), ("RAAR", 0.87), ("ER", 0.0)])
 @pytest.mark.parametrize("reality", [0.0, 0.5])
 def test_one_iteration(hip, orc, n, P, alg, beta, reality):

@@ -30,7 +30,7 @@ def check_mask(got, want, b, thr, tag):
     assert ambiguous.sum() <= 4
 
 
-@pytest.mark.parametrize("n,P", [(128, 6), (512, 3)])
+@pytest.mark.parametrize("n,P", [(128, 6), (384, 2), (512, 3)])
 @pytest.mark.parametrize("sigma,thr", [(3.0, 0.2), (1.5, 0.07)])
 def test_sw(hip, orc, n, P, sigma, thr):
     amp, rho, supp = case(n, P, seed=11)
@@ -45,7 +45,7 @@ def test_sw(hip, orc, n, P, sigma, thr):
         check_mask(got_S[i], want, b, thr * b.max(), f"SW n={n} i={i}")
 
 
-@pytest.mark.parametrize("n,P", [(128, 4), (512, 2)])
+@pytest.mark.parametrize("n,P", [(128, 4), (512, 2), (768, 1)])
 @pytest.mark.parametrize("pixels", [0, 1, 500, 4000, 10 ** 7])
 def test_swa(hip, orc, n, P, pixels):
     amp, rho, supp = case(n, P, seed=12)
@@ -66,7 +66,7 @@ def test_swa(hip, orc, n, P, pixels):
         check_mask(got_S[i], want, b, thr, f"SWA n={n} k={k} i={i}")
 
 
-@pytest.mark.parametrize("n,P", [(128, 5), (512, 3)])
+@pytest.mark.parametrize("n,P", [(128, 5), (384, 2), (512, 3), (768, 2)])
 @pytest.mark.parametrize("reality", [0.0, 0.5])
 def test_evaluate(hip, orc, n, P, reality):
     amp, rho, supp = case(n, P, seed=13)

@@ -80,8 +80,9 @@ def test_sequence_matches_oracle(pr, orc):
     assert one is recs[0]
 
 
-def test_reshift_matches_oracle(pr, orc):
-    n, P = 128, 5
+@pytest.mark.parametrize("n", [128, 384])
+def test_reshift_matches_oracle(pr, orc, n):
+    P = 5
     S, I, recs = population(pr, n, P)
     amp = S.get_pattern()
     # make the individuals shifted / twinned copies of one density
@@ -127,9 +128,9 @@ def test_average_and_average_module(pr, orc):
             assert rel_l2(m, wm) < 1e-6
 
 
-@pytest.mark.parametrize("crossft,use_average", [(1.0, 0.0), (0.0, 0.0), (1.0, 0.5)])
-def test_crossover_matches_oracle(pr, orc, crossft, use_average):
-    n, P = 128, 6
+@pytest.mark.parametrize("n,crossft,use_average", [(128, 1.0, 0.0), (128, 0.0, 0.0), (128, 1.0, 0.5), (384, 1.0, 0.0)])
+def test_crossover_matches_oracle(pr, orc, n, crossft, use_average):
+    P = 6
     S, I, recs = population(pr, n, P, nthreads=P)
     S.set_sequence([("HIO", [5.0, 0.9]), ("ER", [3.0])])
     S.Sequence(recs)
