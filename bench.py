@@ -214,6 +214,67 @@ def time_to_rf_study(seeds, generations, target, log, threshold=None):
     print(json.dumps(out), flush=True)
 
 
+def batch_bench(args, log):
+    """Mixed-size batch: `args.batch` independent polygon patterns with sizes
+    drawn from {256, 384, 512, 768, 1024} and `args.batch_pop` individuals each,
+    all resident in HBM; one step = one RAAR block of `args.batch_block`
+    iterations over every individual of every pattern (one HIP graph launch per
+    block, or direct launches with --no-graphs). One JSON line."""
+    from s_spi_spring_amd import hipcore, synth
+    P = args.batch_pop
+    log(f"generating {args.batch} patterns")
+    amps, starts = [], {}
+    t0 = time.perf_counter()
+    for I, _ in synth.batch_patterns(args.batch, seed=0):
+        amps.append(synth.corner_amplitude(I))
+    sizes = [a.shape[0] for a in amps]
+    log(f"generated in {time.perf_counter() - t0:.1f} s; sizes: " +
+        ", ".join(f"{n}^2 x {sizes.count(n)}" for n in sorted(set(sizes))))
+    b = hipcore.Batch(amps, [P] * len(amps), device=0)
+    b.graphs(not args.no_graphs)
+    for k, a in enumerate(amps):
+        n = a.shape[0]
+        if n not in starts:   # one random start per size, scaled per pattern
+            supp = synth.disk_support(n, max(8, int(0.3 * n)))
+            starts[n] = (synth.random_start(n, P, supp, seed=1), supp)
+        rho, supp = starts[n]
+        scale = float(np.sqrt((np.maximum(a, 0).astype(np.float64) ** 2).sum()) / supp.sum())
+        b.set_population(k, rho * np.complex64(scale), supp)
+    block = [("RAAR", [args.batch_block, args.beta])]
+    log("warmup")
+    for _ in range(args.warmup):
+        b.run(block)
+    log(f"timed: {args.steps} blocks of {args.batch_block} RAAR iterations over {b.total} individuals")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.run(block)
+    dt = time.perf_counter() - t0
+    ind_iters = args.steps * args.batch_block * b.total
+    value = ind_iters / dt
+    # algorithmic bytes of one iteration of the whole batch: 49 B/px per individual
+    # (16 column + 33 row) and each pattern's amplitude once per column pass
+    it_bytes = sum(49.0 * n * n * P + 4.0 * n * n for n in sizes)
+    frac = it_bytes * (value / b.total) / 1e9 / HBM_PEAK_GBS
+    log(f"{value:.0f} individual-iterations/s; whole iteration {it_bytes * value / b.total / 1e9:.0f} GB/s "
+        f"algorithmic ({frac:.3f} of HBM peak)")
+    ev = b.evaluate(0.0)
+    ef = np.concatenate(ev["error_fourier"])
+    out = {"metric": "individual-iters/s, mixed-size pattern batch", "value": round(value, 1),
+           "unit": "individual-iters/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"{len(amps)} polygon patterns of sizes drawn from 256^2..1024^2, population {P} "
+                                  f"each, RAAR(beta={args.beta}) blocks of {args.batch_block} iterations, "
+                                  f"{'one HIP graph launch' if not args.no_graphs else 'direct launches'} per block",
+                      "patterns": len(amps), "sizes": {str(n): sizes.count(n) for n in sorted(set(sizes))},
+                      "population": P, "individuals": b.total, "iterations_per_step": args.batch_block,
+                      "graphs": not args.no_graphs},
+           "iteration_frac": round(frac, 4),
+           "fourier_error_median_after": float(np.median(ef))}
+    print(json.dumps(out), flush=True)
+    b.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,6 +298,11 @@ def main():
     ap.add_argument("--generations", type=int, default=10)
     ap.add_argument("--sw-threshold", type=float, default=None,
                     help="shrink-wrap threshold of the time-to-R_F schedules (default 0.03)")
+    ap.add_argument("--batch", type=int, default=0, metavar="PATTERNS",
+                    help="mixed-size batch workload of PATTERNS patterns (256 in the batching configuration)")
+    ap.add_argument("--batch-pop", type=int, default=32)
+    ap.add_argument("--batch-block", type=int, default=50)
+    ap.add_argument("--no-graphs", action="store_true", help="batch: direct launches instead of HIP graphs")
     ap.add_argument("--summarize-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write the HBM traffic summary from two rocprofv3 --pmc runs of "
                          "`bench.py --calibrate MIB` (FETCH_SIZE, WRITE_SIZE) and exit; no GPU needed")
@@ -256,6 +322,10 @@ def main():
 
     if args.time_to_rf:
         time_to_rf_study(args.time_to_rf, args.generations, args.target, log, args.sw_threshold)
+        return
+
+    if args.batch:
+        batch_bench(args, log)
         return
 
     from s_spi_spring_amd import hipcore, synth

@@ -260,6 +260,37 @@ int spr_timing_read(spr_ctx* ctx, int kind, double* total_ms, long* count);
 /* HIP stream of the handle (a hipStream_t). */
 int spr_stream(spr_ctx* ctx, void** stream);
 
+/* ---------------------------------------------------------------------------
+ * Batches of independent patterns of several sizes on one device.
+ *
+ * No counterpart in the reference, which reconstructs one pattern per Solver
+ * (src/solver.cpp:9-52) and runs several patterns as separate processes. A
+ * batch holds `npatterns` patterns (sizes n[i], any supported size, mixed
+ * freely), each with population[i] >= 1 individuals. Individuals of all
+ * patterns of one size share one set of launches: every column and row pass
+ * covers the whole size class, reading each individual's own amplitude.
+ * spr_batch_run applies the op list of spr_run to every individual of every
+ * pattern; an ER/HIO/RAAR block runs as one HIP graph launch whose branches
+ * (one per size) run concurrently, captured on first use of that block's
+ * (algorithm, iterations, beta, reality) and replayed afterwards;
+ * spr_batch_graphs(b, 0) issues the same launches directly instead.
+ * Populations start at rho = 0 and empty supports; spr_batch_set_population
+ * and spr_batch_get_population copy one pattern's population[i] individuals
+ * (layouts as spr_set_population). spr_batch_evaluate fills the metrics of
+ * spr_evaluate (no symmetry) for all individuals, pattern by pattern, in
+ * arrays of sum(population) entries. */
+typedef struct spr_batch spr_batch;
+int spr_batch_create(spr_batch** out, int npatterns, const int* n, const int* population, const float* const* amp,
+                     int device);
+int spr_batch_destroy(spr_batch* b);
+const char* spr_batch_last_error(const spr_batch* b);
+int spr_batch_graphs(spr_batch* b, int enable);
+int spr_batch_set_population(spr_batch* b, int pattern, const float* rho, const int32_t* support);
+int spr_batch_get_population(spr_batch* b, int pattern, float* rho, int32_t* support);
+int spr_batch_run(spr_batch* b, const spr_op* ops, int nops, int nrep, float reality);
+int spr_batch_evaluate(spr_batch* b, float reality, float* error_fourier, float* error_real, float* complexsum,
+                       float* modulesum);
+
 #ifdef __cplusplus
 }
 #endif

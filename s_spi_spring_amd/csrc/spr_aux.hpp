@@ -277,7 +277,8 @@ enum AuxColMode {
 struct AuxColArgs {
     float2* f;
     float2* f2;
-    const float* amp;
+    const float* amp;           // amplitudes [patterns][N][N]
+    const int* amp_of;          // item -> pattern (NULL: pattern 0)
     const float2* tw;
     float sigma_ft;
     double* partial;
@@ -304,6 +305,7 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_aux(AuxColArgs a) {
     const int item = strip / TILES;
     const int x0 = (strip % TILES) * C;
     const size_t base = (size_t)item * N * N + x0 + c;
+    const float* __restrict__ amp = a.amp + amp_offset<N>(a.amp_of, item);
     const ColAcc<G::CS> acc{lds + c};
     float2 v[V];
 #pragma unroll
@@ -325,7 +327,7 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_aux(AuxColArgs a) {
         float fmax = 0.f;
 #pragma unroll
         for (int q = 0; q < V; ++q) {
-            const float A = a.amp[(size_t)(t + q * T) * N + x0 + c];
+            const float A = amp[(size_t)(t + q * T) * N + x0 + c];
             const float m = sqrtf(v[q].y * v[q].y + v[q].x * v[q].x);
             fmax = vmax(fmax, m);
             if (A >= 0.f) {
@@ -379,7 +381,7 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_aux(AuxColArgs a) {
                 const float w = corr_weight<N>(x0 + c, ky, a.win);
                 cx *= w;
                 cy *= w;
-                if (a.amp[(size_t)ky * N + x0 + c] < 0.f) {
+                if (amp[(size_t)ky * N + x0 + c] < 0.f) {
                     cx = 0.f;
                     cy = 0.f;
                 }

@@ -7,7 +7,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "spring_hip.h"
@@ -30,8 +32,10 @@ struct spr_ctx {
     float2* work = nullptr;
     float2* work2 = nullptr;      // second half-transformed field per individual (correlations, |b|)
     uint8_t* supp = nullptr;
-    float* amp = nullptr;
+    float* amp = nullptr;         // amplitudes [npat][n][n]
     float* amp_t = nullptr;       // amp transposed (read along columns by the column pass)
+    int npat = 1;                 // patterns sharing this handle (one amplitude each)
+    int* amp_of = nullptr;        // slot -> pattern when npat > 1, else NULL
     float2* tw = nullptr;
     // reduction scratch
     double* partial = nullptr;    // per workgroup partial sums
@@ -158,7 +162,7 @@ template <int N> int launch_row(spr_ctx* c, int mode, const RowArgs& a, int cnt,
 }
 
 template <int N> int launch_col(spr_ctx* c, int mode, float2* f, int cnt, hipStream_t s) {
-    const ColArgs a{f, c->amp, c->tw, c->amp_t};
+    const ColArgs a{f, c->amp, c->tw, c->amp_t, c->amp_of};
     const dim3 grid((unsigned)((size_t)cnt * (N / Geo<N>::C))), block(Geo<N>::COL_THREADS);
     switch (mode) {
         case COL_PROJ: hipLaunchKernelGGL((k_col<N, COL_PROJ>), grid, block, 0, s, a); break;
@@ -303,6 +307,7 @@ AuxColArgs aux_col_args(spr_ctx* c) {
     a.f = c->work;
     a.f2 = c->work2;
     a.amp = c->amp;
+    a.amp_of = c->amp_of;
     a.tw = c->tw;
     a.partial = c->partial;
     a.maxbits = c->maxbits;
@@ -649,7 +654,7 @@ int queue_adam_steps(spr_ctx* c, int steps, float lr, const float* lrp, int kind
     const float b1 = 0.9f, b2 = 0.999f;
     float p1 = 1.f, p2 = 1.f;
     for (int s = 1; s <= steps; ++s) {
-        LossArgs la{c->work, c->amp, c->tw, c->partial, eps, kind};
+        LossArgs la{c->work, c->amp, c->amp_of, c->tw, c->partial, eps, kind};
         SPR_TRY2(col_loss(c, la, P));
         p1 *= b1;
         p2 *= b2;
@@ -676,7 +681,7 @@ int queue_loss_grad(spr_ctx* c, int P, const float2* rho, float2* grad, int kind
     const int n = c->n;
     FieldBatch fwd{c->work, nullptr, c->tw, 1.f, rho};
     SPR_TRY2(fft_rows(c, +1, fwd, P));
-    LossArgs la{c->work, c->amp, c->tw, c->partial, eps, kind};
+    LossArgs la{c->work, c->amp, c->amp_of, c->tw, c->partial, eps, kind};
     SPR_TRY2(col_loss(c, la, P));
     SPR_TRY2(reduce(c, n / cols_per_wg(n), 1, 7, P));
     FieldBatch adj{grad, nullptr, c->tw, 2.f, c->work};
@@ -711,12 +716,23 @@ int spr_supported_size(int n) {
     return 0;
 }
 
-int spr_create(spr_ctx** out, int n, int capacity, const float* amp, int device) {
+// A handle for `npat` patterns of size n: amps[k] is pattern k's corner-origin
+// amplitude and amp_of (capacity entries) gives every slot's pattern (NULL for a
+// single pattern).
+static int create_context(spr_ctx** out, int n, int capacity, int npat, const float* const* amps, const int* amp_of,
+                          int device) {
     if (!out) return fail(nullptr, SPR_ERR_ARG, "out is NULL");
     *out = nullptr;
     if (!spr_supported_size(n)) return fail(nullptr, SPR_ERR_ARG, "unsupported grid size " + std::to_string(n));
     if (capacity < 1) return fail(nullptr, SPR_ERR_ARG, "capacity must be >= 1");
-    if (!amp) return fail(nullptr, SPR_ERR_ARG, "amp is NULL");
+    if (npat < 1 || !amps) return fail(nullptr, SPR_ERR_ARG, "amp is NULL");
+    for (int k = 0; k < npat; ++k)
+        if (!amps[k]) return fail(nullptr, SPR_ERR_ARG, "amp is NULL");
+    if (npat > 1) {
+        if (!amp_of) return fail(nullptr, SPR_ERR_ARG, "pattern map is NULL");
+        for (int i = 0; i < capacity; ++i)
+            if (amp_of[i] < 0 || amp_of[i] >= npat) return fail(nullptr, SPR_ERR_ARG, "pattern map out of range");
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(nullptr, SPR_ERR_HIP, "no HIP device");
     if (device < 0 || device >= ndev) return fail(nullptr, SPR_ERR_ARG, "bad device id");
@@ -724,6 +740,7 @@ int spr_create(spr_ctx** out, int n, int capacity, const float* amp, int device)
     c->n = n;
     c->cap = capacity;
     c->device = device;
+    c->npat = npat;
     const size_t np = (size_t)n * n;
     auto cleanup_fail = [&](const char* what, hipError_t e) {
         g_create_error = std::string(what) + ": " + hipGetErrorString(e);
@@ -746,8 +763,8 @@ int spr_create(spr_ctx** out, int n, int capacity, const float* amp, int device)
     if ((e = hipMalloc(&c->work, np * capacity * sizeof(float2))) != hipSuccess)
         return cleanup_fail("hipMalloc work", e);
     if ((e = hipMalloc(&c->supp, np * capacity)) != hipSuccess) return cleanup_fail("hipMalloc supp", e);
-    if ((e = hipMalloc(&c->amp, np * sizeof(float))) != hipSuccess) return cleanup_fail("hipMalloc amp", e);
-    if ((e = hipMalloc(&c->amp_t, np * sizeof(float))) != hipSuccess) return cleanup_fail("hipMalloc amp_t", e);
+    if ((e = hipMalloc(&c->amp, np * npat * sizeof(float))) != hipSuccess) return cleanup_fail("hipMalloc amp", e);
+    if ((e = hipMalloc(&c->amp_t, np * npat * sizeof(float))) != hipSuccess) return cleanup_fail("hipMalloc amp_t", e);
     if ((e = hipMalloc(&c->tw, n * sizeof(float2))) != hipSuccess) return cleanup_fail("hipMalloc tw", e);
     if ((e = hipMalloc(&c->work2, np * capacity * sizeof(float2))) != hipSuccess)
         return cleanup_fail("hipMalloc work2", e);
@@ -767,6 +784,12 @@ int spr_create(spr_ctx** out, int n, int capacity, const float* amp, int device)
         return cleanup_fail("hipMalloc radix", e);
     if ((e = hipMalloc(&c->hist, (size_t)capacity * RADIX_BINS * sizeof(unsigned))) != hipSuccess)
         return cleanup_fail("hipMalloc hist", e);
+    if (npat > 1) {
+        if ((e = hipMalloc(&c->amp_of, (size_t)capacity * sizeof(int))) != hipSuccess)
+            return cleanup_fail("hipMalloc pattern map", e);
+        if ((e = hipMemcpy(c->amp_of, amp_of, (size_t)capacity * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+            return cleanup_fail("upload pattern map", e);
+    }
     std::vector<float2> tw(n);
     for (int k = 0; k < n; ++k) {
         const double ang = -2.0 * M_PI * (double)k / (double)n;
@@ -774,16 +797,21 @@ int spr_create(spr_ctx** out, int n, int capacity, const float* amp, int device)
     }
     if ((e = hipMemcpy(c->tw, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice)) != hipSuccess)
         return cleanup_fail("upload twiddles", e);
-    if ((e = hipMemcpy(c->amp, amp, np * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
-        return cleanup_fail("upload amplitude", e);
-    {
-        const std::vector<float> at = transpose_amp(amp, n);
-        if ((e = hipMemcpy(c->amp_t, at.data(), np * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
+    for (int k = 0; k < npat; ++k) {
+        if ((e = hipMemcpy(c->amp + k * np, amps[k], np * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
+            return cleanup_fail("upload amplitude", e);
+        const std::vector<float> at = transpose_amp(amps[k], n);
+        if ((e = hipMemcpy(c->amp_t + k * np, at.data(), np * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
             return cleanup_fail("upload amplitude", e);
     }
     if ((e = hipMemset(c->supp, 0, np * capacity)) != hipSuccess) return cleanup_fail("clear supports", e);
     *out = c;
     return SPR_OK;
+}
+
+int spr_create(spr_ctx** out, int n, int capacity, const float* amp, int device) {
+    const float* amps[1] = {amp};
+    return create_context(out, n, capacity, 1, amps, nullptr, device);
 }
 
 int spr_destroy(spr_ctx* c) {
@@ -796,6 +824,7 @@ int spr_destroy(spr_ctx* c) {
     (void)hipFree(c->supp);
     (void)hipFree(c->amp);
     (void)hipFree(c->amp_t);
+    (void)hipFree(c->amp_of);
     (void)hipFree(c->tw);
     (void)hipFree(c->work2);
     (void)hipFree(c->partial);
@@ -827,6 +856,7 @@ int spr_size(const spr_ctx* c, int* n, int* capacity) {
 
 int spr_set_pattern(spr_ctx* c, const float* amp) {
     if (!c || !amp) return fail(c, SPR_ERR_ARG, "NULL argument");
+    if (c->npat != 1) return fail(c, SPR_ERR_ARG, "the handle holds several patterns");
     SPR_HIP(c, hipSetDevice(c->device));
     const std::vector<float> at = transpose_amp(amp, c->n);
     SPR_HIP(c, hipMemcpyAsync(c->amp, amp, (size_t)c->n * c->n * sizeof(float), hipMemcpyHostToDevice, c->stream));
@@ -838,6 +868,7 @@ int spr_set_pattern(spr_ctx* c, const float* amp) {
 
 int spr_get_pattern(spr_ctx* c, float* amp) {
     if (!c || !amp) return fail(c, SPR_ERR_ARG, "NULL argument");
+    if (c->npat != 1) return fail(c, SPR_ERR_ARG, "the handle holds several patterns");
     SPR_HIP(c, hipSetDevice(c->device));
     SPR_HIP(c, hipMemcpyAsync(amp, c->amp, (size_t)c->n * c->n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     SPR_HIP(c, hipStreamSynchronize(c->stream));
@@ -1519,6 +1550,313 @@ int spr_device_buffers(spr_ctx* c, void** rho, void** support_u8, void** amp) {
 int spr_stream(spr_ctx* c, void** stream) {
     if (!c || !stream) return fail(c, SPR_ERR_ARG, "NULL argument");
     *stream = (void*)c->stream;
+    return SPR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batches of patterns of several sizes (spring_hip.h, spr_batch_*)
+// ---------------------------------------------------------------------------
+struct spr_batch {
+    int device = 0;
+    struct Class {
+        int n = 0;
+        spr_ctx* ctx = nullptr;       // every individual of every pattern of size n; its stream is the class's
+        hipEvent_t done = nullptr;
+    };
+    std::vector<Class> classes;
+    std::vector<int> pat_class, pat_first, pat_pop;   // per pattern: class, first slot, population
+    hipStream_t origin = nullptr;                     // stream the block graphs are launched on
+    hipEvent_t fork = nullptr;
+    bool graphs = true;
+    std::map<std::tuple<int, int, unsigned, unsigned>, hipGraphExec_t> blocks;
+    std::string err;
+};
+
+static int bfail(spr_batch* b, int code, const std::string& msg) {
+    if (b) b->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define SPR_BHIP(b, call)                                                                           \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return bfail((b), SPR_ERR_HIP, std::string(#call " failed: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// A class handle's error becomes the batch's.
+static int bclass(spr_batch* b, spr_batch::Class& k, int rc) {
+    if (rc != SPR_OK) b->err = "size " + std::to_string(k.n) + ": " + k.ctx->err;
+    return rc;
+}
+
+static int batch_sync(spr_batch* b) {
+    SPR_BHIP(b, hipSetDevice(b->device));
+    for (auto& k : b->classes) SPR_BHIP(b, hipStreamSynchronize(k.ctx->stream));
+    SPR_BHIP(b, hipStreamSynchronize(b->origin));
+    return SPR_OK;
+}
+
+// Every class stream's pending work before the origin stream's next launch, and
+// the reverse: `to_origin` true makes the origin wait for the classes.
+static int batch_join(spr_batch* b, bool to_origin) {
+    if (to_origin) {
+        for (auto& k : b->classes) {
+            SPR_BHIP(b, hipEventRecord(k.done, k.ctx->stream));
+            SPR_BHIP(b, hipStreamWaitEvent(b->origin, k.done, 0));
+        }
+    } else {
+        SPR_BHIP(b, hipEventRecord(b->fork, b->origin));
+        for (auto& k : b->classes) SPR_BHIP(b, hipStreamWaitEvent(k.ctx->stream, b->fork, 0));
+    }
+    return SPR_OK;
+}
+
+// The iteration block (alg, niter, beta, reality) of every class as one graph:
+// captured once from the origin stream, forking to the class streams (one
+// branch per size) and joining back.
+static int batch_block_graph(spr_batch* b, int alg, int niter, float beta, float reality, hipGraphExec_t* out) {
+    unsigned bb, rb;
+    std::memcpy(&bb, &beta, 4);
+    std::memcpy(&rb, &reality, 4);
+    const auto key = std::make_tuple(alg, niter, bb, rb);
+    auto it = b->blocks.find(key);
+    if (it != b->blocks.end()) {
+        *out = it->second;
+        return SPR_OK;
+    }
+    SPR_BHIP(b, hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
+    int rc = SPR_OK;
+    if (hipEventRecord(b->fork, b->origin) != hipSuccess) rc = bfail(b, SPR_ERR_HIP, "capture fork failed");
+    for (size_t i = 0; rc == SPR_OK && i < b->classes.size(); ++i) {
+        auto& k = b->classes[i];
+        if (hipStreamWaitEvent(k.ctx->stream, b->fork, 0) != hipSuccess) {
+            rc = bfail(b, SPR_ERR_HIP, "capture fork failed");
+            break;
+        }
+        rc = bclass(b, k, queue_iterate(k.ctx, alg, niter, beta, reality));
+        if (rc == SPR_OK && (hipEventRecord(k.done, k.ctx->stream) != hipSuccess ||
+                             hipStreamWaitEvent(b->origin, k.done, 0) != hipSuccess))
+            rc = bfail(b, SPR_ERR_HIP, "capture join failed");
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->origin, &g);
+    if (rc != SPR_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return bfail(b, SPR_ERR_HIP, std::string("hipStreamEndCapture failed: ") + hipGetErrorString(e));
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess)
+        return bfail(b, SPR_ERR_HIP, std::string("hipGraphInstantiate failed: ") + hipGetErrorString(ei));
+    b->blocks[key] = x;
+    *out = x;
+    return SPR_OK;
+}
+
+static int batch_iterate(spr_batch* b, int alg, int niter, float beta, float reality) {
+    if (!b->graphs) {
+        for (auto& k : b->classes) SPR_TRY(bclass(b, k, queue_iterate(k.ctx, alg, niter, beta, reality)));
+        return SPR_OK;
+    }
+    hipGraphExec_t x = nullptr;
+    SPR_TRY(batch_join(b, true));
+    SPR_TRY(batch_block_graph(b, alg, niter, beta, reality, &x));
+    SPR_BHIP(b, hipGraphLaunch(x, b->origin));
+    return batch_join(b, false);
+}
+
+int spr_batch_create(spr_batch** out, int npatterns, const int* n, const int* population, const float* const* amp,
+                     int device) {
+    if (!out) return bfail(nullptr, SPR_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (npatterns < 1 || !n || !population || !amp) return bfail(nullptr, SPR_ERR_ARG, "NULL or empty pattern list");
+    for (int i = 0; i < npatterns; ++i) {
+        if (!spr_supported_size(n[i]))
+            return bfail(nullptr, SPR_ERR_ARG, "pattern " + std::to_string(i) + ": unsupported grid size " +
+                                                   std::to_string(n[i]));
+        if (population[i] < 1) return bfail(nullptr, SPR_ERR_ARG, "pattern " + std::to_string(i) + ": empty population");
+        if (!amp[i]) return bfail(nullptr, SPR_ERR_ARG, "pattern " + std::to_string(i) + ": amp is NULL");
+    }
+    spr_batch* b = new spr_batch();
+    b->device = device;
+    b->pat_class.assign(npatterns, -1);
+    b->pat_first.assign(npatterns, 0);
+    b->pat_pop.assign(population, population + npatterns);
+    // Classes by size, in order of first appearance; patterns keep their order within a class.
+    std::vector<int> sizes;
+    for (int i = 0; i < npatterns; ++i)
+        if (std::find(sizes.begin(), sizes.end(), n[i]) == sizes.end()) sizes.push_back(n[i]);
+    auto undo = [&](int rc) {
+        spr_batch_destroy(b);
+        return rc;
+    };
+    for (int size : sizes) {
+        std::vector<const float*> amps;
+        std::vector<int> amp_of;
+        for (int i = 0; i < npatterns; ++i) {
+            if (n[i] != size) continue;
+            b->pat_class[i] = (int)b->classes.size();
+            b->pat_first[i] = (int)amp_of.size();
+            for (int j = 0; j < population[i]; ++j) amp_of.push_back((int)amps.size());
+            amps.push_back(amp[i]);
+        }
+        spr_batch::Class k;
+        k.n = size;
+        const int cap = (int)amp_of.size();
+        const int rc = create_context(&k.ctx, size, cap, (int)amps.size(), amps.data(),
+                                      amps.size() > 1 ? amp_of.data() : nullptr, device);
+        if (rc != SPR_OK) {
+            g_create_error = "size " + std::to_string(size) + ": " + g_create_error;
+            return undo(rc);
+        }
+        b->classes.push_back(k);
+        spr_batch::Class& kk = b->classes.back();
+        kk.ctx->lanes = 1;
+        kk.ctx->P = cap;
+        if (hipMemset(kk.ctx->rho, 0, (size_t)size * size * cap * sizeof(float2)) != hipSuccess ||
+            hipEventCreateWithFlags(&kk.done, hipEventDisableTiming) != hipSuccess) {
+            g_create_error = "batch set-up failed";
+            return undo(SPR_ERR_HIP);
+        }
+    }
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) {
+        g_create_error = "batch stream set-up failed";
+        return undo(SPR_ERR_HIP);
+    }
+    *out = b;
+    return SPR_OK;
+}
+
+int spr_batch_destroy(spr_batch* b) {
+    if (!b) return SPR_OK;
+    (void)hipSetDevice(b->device);
+    if (b->origin) (void)hipStreamSynchronize(b->origin);
+    for (auto& kv : b->blocks) (void)hipGraphExecDestroy(kv.second);
+    for (auto& k : b->classes) {
+        if (k.done) (void)hipEventDestroy(k.done);
+        spr_destroy(k.ctx);
+    }
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->origin) (void)hipStreamDestroy(b->origin);
+    delete b;
+    return SPR_OK;
+}
+
+const char* spr_batch_last_error(const spr_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int spr_batch_graphs(spr_batch* b, int enable) {
+    if (!b) return bfail(b, SPR_ERR_ARG, "NULL handle");
+    b->graphs = enable != 0;
+    return SPR_OK;
+}
+
+static int batch_pattern(spr_batch* b, int pattern) {
+    if (!b) return bfail(b, SPR_ERR_ARG, "NULL handle");
+    if (pattern < 0 || pattern >= (int)b->pat_class.size()) return bfail(b, SPR_ERR_ARG, "pattern index out of range");
+    return SPR_OK;
+}
+
+int spr_batch_set_population(spr_batch* b, int pattern, const float* rho, const int32_t* support) {
+    SPR_TRY(batch_pattern(b, pattern));
+    if (!rho && !support) return bfail(b, SPR_ERR_ARG, "rho and support are NULL");
+    auto& k = b->classes[b->pat_class[pattern]];
+    spr_ctx* c = k.ctx;
+    const size_t np = (size_t)k.n * k.n, P = (size_t)b->pat_pop[pattern], first = (size_t)b->pat_first[pattern];
+    SPR_TRY(batch_sync(b));
+    if (rho) SPR_BHIP(b, hipMemcpy(c->rho + first * np, rho, np * P * sizeof(float2), hipMemcpyHostToDevice));
+    if (support) {
+        std::vector<uint8_t> s(np * P);
+        for (size_t i = 0; i < s.size(); ++i) s[i] = support[i] != 0 ? 1 : 0;
+        SPR_BHIP(b, hipMemcpy(c->supp + first * np, s.data(), s.size(), hipMemcpyHostToDevice));
+    }
+    return SPR_OK;
+}
+
+int spr_batch_get_population(spr_batch* b, int pattern, float* rho, int32_t* support) {
+    SPR_TRY(batch_pattern(b, pattern));
+    auto& k = b->classes[b->pat_class[pattern]];
+    spr_ctx* c = k.ctx;
+    const size_t np = (size_t)k.n * k.n, P = (size_t)b->pat_pop[pattern], first = (size_t)b->pat_first[pattern];
+    SPR_TRY(batch_sync(b));
+    if (rho) SPR_BHIP(b, hipMemcpy(rho, c->rho + first * np, np * P * sizeof(float2), hipMemcpyDeviceToHost));
+    if (support) {
+        std::vector<uint8_t> s(np * P);
+        SPR_BHIP(b, hipMemcpy(s.data(), c->supp + first * np, s.size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < s.size(); ++i) support[i] = s[i];
+    }
+    return SPR_OK;
+}
+
+int spr_batch_run(spr_batch* b, const spr_op* ops, int nops, int nrep, float reality) {
+    if (!b) return bfail(b, SPR_ERR_ARG, "NULL handle");
+    if (nops < 0 || nrep < 0 || (nops > 0 && !ops)) return bfail(b, SPR_ERR_ARG, "bad op list");
+    for (int i = 0; i < nops; ++i) {
+        const spr_op& o = ops[i];
+        if (o.kind < SPR_OP_ER || o.kind > SPR_OP_GRAD) return bfail(b, SPR_ERR_ARG, "unknown op kind");
+        if (o.kind <= SPR_OP_RAAR && o.iterations < 0) return bfail(b, SPR_ERR_ARG, "negative iteration count");
+    }
+    SPR_BHIP(b, hipSetDevice(b->device));
+    for (int r = 0; r < nrep; ++r) {
+        for (int i = 0; i < nops; ++i) {
+            const spr_op& o = ops[i];
+            switch (o.kind) {
+                case SPR_OP_ER: SPR_TRY(batch_iterate(b, SPR_ER, o.iterations, 0.f, reality)); break;
+                case SPR_OP_HIO: SPR_TRY(batch_iterate(b, SPR_HIO, o.iterations, o.beta, reality)); break;
+                case SPR_OP_RAAR: SPR_TRY(batch_iterate(b, SPR_RAAR, o.iterations, o.beta, reality)); break;
+                case SPR_OP_SW:
+                    for (auto& k : b->classes) SPR_TRY(bclass(b, k, queue_sw(k.ctx, o.sigma, o.threshold)));
+                    break;
+                case SPR_OP_SWA:
+                    for (auto& k : b->classes) SPR_TRY(bclass(b, k, queue_swa(k.ctx, o.sigma, o.pixels)));
+                    break;
+                case SPR_OP_GRAD:
+                    for (auto& k : b->classes)
+                        SPR_TRY(bclass(b, k, spr_refine_support(k.ctx, o.iterations, o.beta, o.pixels, o.threshold,
+                                                                nullptr)));
+                    break;
+            }
+        }
+    }
+    return batch_sync(b);
+}
+
+int spr_batch_evaluate(spr_batch* b, float reality, float* error_fourier, float* error_real, float* complexsum,
+                       float* modulesum) {
+    if (!b) return bfail(b, SPR_ERR_ARG, "NULL handle");
+    SPR_BHIP(b, hipSetDevice(b->device));
+    SPR_TRY(batch_sync(b));
+    for (auto& k : b->classes) SPR_TRY(bclass(b, k, queue_error(k.ctx, reality)));
+    std::vector<std::vector<double>> sums(b->classes.size());
+    std::vector<std::vector<unsigned>> mx(b->classes.size());
+    for (size_t i = 0; i < b->classes.size(); ++i) {
+        spr_ctx* c = b->classes[i].ctx;
+        sums[i].resize((size_t)c->P * SUMS_PER_ITEM);
+        mx[i].resize(c->P);
+        SPR_BHIP(b, hipMemcpyAsync(sums[i].data(), c->sums, sums[i].size() * sizeof(double), hipMemcpyDeviceToHost,
+                                   c->stream));
+        SPR_BHIP(b, hipMemcpyAsync(mx[i].data(), c->maxbits, mx[i].size() * sizeof(unsigned), hipMemcpyDeviceToHost,
+                                   c->stream));
+    }
+    SPR_TRY(batch_sync(b));
+    size_t out = 0;
+    for (size_t pat = 0; pat < b->pat_class.size(); ++pat) {
+        const int ci = b->pat_class[pat];
+        for (int j = 0; j < b->pat_pop[pat]; ++j, ++out) {
+            const size_t slot = (size_t)b->pat_first[pat] + j;
+            const double* s = &sums[ci][slot * SUMS_PER_ITEM];
+            float fm;
+            std::memcpy(&fm, &mx[ci][slot], 4);
+            if (error_fourier) error_fourier[out] = (float)std::sqrt(s[1] / s[2]);
+            if (error_real) error_real[out] = (float)std::sqrt(s[3] / s[4]);
+            if (complexsum) complexsum[out] = fm;
+            if (modulesum) modulesum[out] = (float)s[0];
+        }
+    }
     return SPR_OK;
 }
 

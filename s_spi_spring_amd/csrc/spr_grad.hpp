@@ -19,7 +19,8 @@ enum LossKind { LOSS_POISSON = 0, LOSS_AMPLITUDE = 1 };
 
 struct LossArgs {
     float2* f;           // row-transformed fields (in), adjoint-column-transformed weights (out)
-    const float* amp;
+    const float* amp;    // amplitudes [patterns][N][N]
+    const int* amp_of;   // field -> pattern (NULL: pattern 0)
     const float2* tw;
     double* partial;     // one loss partial per workgroup
     float eps;
@@ -39,7 +40,7 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_loss(LossArgs a) {
     const int item = strip / TILES;
     const int x0 = (strip % TILES) * C;
     float2* __restrict__ f = a.f + (size_t)item * N * N + x0;
-    const float* __restrict__ amp = a.amp + x0;
+    const float* __restrict__ amp = a.amp + amp_offset<N>(a.amp_of, item) + x0;
     const int off = t * N + c;
     const ColAcc<G::CS> acc{lds + c};
     float2 v[V];

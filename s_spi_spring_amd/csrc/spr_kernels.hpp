@@ -308,10 +308,16 @@ enum ColMode {
 
 struct ColArgs {
     float2* f;          // field batch, transformed in place
-    const float* amp;
+    const float* amp;   // amplitudes [patterns][N][N]
     const float2* tw;
-    const float* amp_t; // the amplitude transposed, amp_t[x*N + y] = amp[y*N + x]
+    const float* amp_t; // the amplitudes transposed, amp_t[x*N + y] = amp[y*N + x]
+    const int* amp_of;  // field -> pattern (NULL: every field belongs to pattern 0)
 };
+
+// Offset of the amplitude of field p (block-uniform: one scalar load).
+template <int N> __device__ __forceinline__ size_t amp_offset(const int* amp_of, int p) {
+    return amp_of ? (size_t)amp_of[p] * N * N : 0;
+}
 
 // Column strip of workgroup `b`. Workgroups are dispatched round-robin over the
 // 8 XCDs (b mod 8), each with its own L2. When a strip is narrower than a
@@ -373,7 +379,7 @@ void k_col(ColArgs a) {
         for (int r = 0; r < V; ++r) v[r] = acc.ldk<T>(tl, r);
         if constexpr (MODE == COL_PROJ || MODE == COL_FWD) fft_line<N, +1, true>(v, acc, tw, tl);
         if constexpr (MODE == COL_PROJ) {
-            const float* __restrict__ at = a.amp_t + (x0 + cc) * N;
+            const float* __restrict__ at = a.amp_t + amp_offset<N>(a.amp_of, p) + (x0 + cc) * N;
 #pragma unroll
             for (int q = 0; q < V; ++q) v[q] = project_modulus(v[q], at[tl + q * T]);
         }
@@ -384,7 +390,7 @@ void k_col(ColArgs a) {
 #pragma unroll
         for (int r = 0; r < V; ++r) v[r] = mine.ldk<T>(t, r);
     } else {
-        const float* __restrict__ amp = a.amp + x0;
+        const float* __restrict__ amp = a.amp + amp_offset<N>(a.amp_of, p) + x0;
         const ColAcc<CS> acc{lds + c};
         float am[V];
         if constexpr (MODE == COL_PROJ && SPR_COL_AMP_PREFETCH) {

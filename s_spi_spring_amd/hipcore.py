@@ -61,6 +61,14 @@ SIGNATURES = {
     "spr_loss_grad": (_I, [_VP, _I, _VP, _VP, _VP, _I, _F, _VP]),
     "spr_refine": (_I, [_VP, _I, _F, _I, _F, _VP]),
     "spr_refine_support": (_I, [_VP, _I, _F, _I, _F, _VP]),
+    "spr_batch_create": (_I, [ctypes.POINTER(_VP), _I, _VP, _VP, _VP, _I]),
+    "spr_batch_destroy": (_I, [_VP]),
+    "spr_batch_last_error": (ctypes.c_char_p, [_VP]),
+    "spr_batch_graphs": (_I, [_VP, _I]),
+    "spr_batch_set_population": (_I, [_VP, _I, _VP, _VP]),
+    "spr_batch_get_population": (_I, [_VP, _I, _VP, _VP]),
+    "spr_batch_run": (_I, [_VP, _VP, _I, _I, _F]),
+    "spr_batch_evaluate": (_I, [_VP, _F, _VP, _VP, _VP, _VP]),
 }
 
 
@@ -317,3 +325,78 @@ class Backend:
         s = ctypes.c_void_p()
         self._check(self.lib.spr_stream(self.h, ctypes.byref(s)))
         return s.value
+
+
+class Batch:
+    """Many patterns of several sizes on one device (spr_batch): every ER/HIO/RAAR
+    block covers all individuals of all patterns, one launch set per size.
+
+    amps: corner-origin amplitudes (n_i x n_i, any supported sizes); populations:
+    individuals per pattern."""
+
+    def __init__(self, amps, populations, device: int = 0):
+        self.lib = load()
+        self.amps = [np.ascontiguousarray(a, dtype=np.float32) for a in amps]
+        for a in self.amps:
+            if a.ndim != 2 or a.shape[0] != a.shape[1]:
+                raise ValueError("every amplitude must be square")
+        self.sizes = np.array([a.shape[0] for a in self.amps], np.int32)
+        self.populations = np.array([int(p) for p in populations], np.int32)
+        if len(self.populations) != len(self.amps):
+            raise ValueError("one population size per pattern")
+        ptrs = (ctypes.c_void_p * len(self.amps))(*[a.ctypes.data for a in self.amps])
+        h = ctypes.c_void_p()
+        rc = self.lib.spr_batch_create(ctypes.byref(h), len(self.amps), _ptr(self.sizes), _ptr(self.populations),
+                                       ctypes.cast(ptrs, ctypes.c_void_p), int(device))
+        if rc != 0:
+            raise SpringHipError(self.lib.spr_batch_last_error(None).decode())
+        self.h = h
+
+    @property
+    def total(self):
+        return int(self.populations.sum())
+
+    def _check(self, rc):
+        if rc != 0:
+            raise SpringHipError(self.lib.spr_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.spr_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def graphs(self, enable: bool):
+        self._check(self.lib.spr_batch_graphs(self.h, 1 if enable else 0))
+
+    def set_population(self, k, rho, support=None):
+        n, P = int(self.sizes[k]), int(self.populations[k])
+        rho = np.ascontiguousarray(np.broadcast_to(rho, (P, n, n)), dtype=np.complex64)
+        s = None if support is None else np.ascontiguousarray(np.broadcast_to(support, (P, n, n)), dtype=np.int32)
+        self._check(self.lib.spr_batch_set_population(self.h, int(k), _ptr(rho), _ptr(s)))
+
+    def get_population(self, k, support=True):
+        n, P = int(self.sizes[k]), int(self.populations[k])
+        rho = np.empty((P, n, n), np.complex64)
+        s = np.empty((P, n, n), np.int32) if support else None
+        self._check(self.lib.spr_batch_get_population(self.h, int(k), _ptr(rho), _ptr(s)))
+        return (rho, s) if support else rho
+
+    def run(self, sequence, nrep=1, reality=0.0):
+        ops = make_ops(sequence)
+        self._check(self.lib.spr_batch_run(self.h, ctypes.cast(ops, ctypes.c_void_p), len(sequence), int(nrep),
+                                           float(reality)))
+
+    def evaluate(self, reality=0.0):
+        """Dict of per-pattern arrays: error_fourier, error_real, complexsum, modulesum."""
+        out = {k: np.zeros(self.total, np.float32) for k in ("error_fourier", "error_real", "complexsum", "modulesum")}
+        self._check(self.lib.spr_batch_evaluate(self.h, float(reality), _ptr(out["error_fourier"]),
+                                                _ptr(out["error_real"]), _ptr(out["complexsum"]),
+                                                _ptr(out["modulesum"])))
+        bounds = np.concatenate([[0], np.cumsum(self.populations)])
+        return {k: [v[bounds[i]:bounds[i + 1]] for i in range(len(self.populations))] for k, v in out.items()}

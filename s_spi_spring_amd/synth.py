@@ -12,6 +12,8 @@ Workloads:
   ellipsoids_pattern   512^2 overlapping projected ellipsoids, 1e9 photons, 12 px beamstop,
                        two 4 px detector-gap strips, 0.5 % random saturated pixels masked
   octahedron_pattern   1024^2 truncated-octahedron silhouette with rough boundary
+  batch_patterns       many polygon patterns of sizes drawn from {256, 384, 512, 768, 1024},
+                       objects scaled by n/128, photons log-uniform in [1e7, 1e10]
 """
 from __future__ import annotations
 
@@ -39,13 +41,15 @@ def _beamstop(n, radius):
     return (x - c) ** 2 + (y - c) ** 2 <= radius ** 2
 
 
-def polygon_pattern(seed=0, n=128, photons=1e7, beamstop=5):
-    """Uniform-density convex polygon (6-12 vertices, radius 15-25 px) with a
-    weak random phase (uniform +-0.3 rad), centred in an n^2 grid."""
+def polygon_pattern(seed=0, n=128, photons=1e7, beamstop=5, scale=1.0):
+    """Uniform-density convex polygon (6-12 vertices, radius 15-25 px times
+    `scale`) with a weak random phase (uniform +-0.3 rad), centred in an n^2
+    grid; the beamstop radius is scaled likewise."""
     rng = np.random.default_rng(seed)
     nv = int(rng.integers(6, 13))
     ang = np.sort(rng.uniform(0, 2 * np.pi, nv))
-    rad = rng.uniform(15, 25, nv)
+    rad = rng.uniform(15, 25, nv) * scale
+    beamstop = int(round(beamstop * scale))
     vx = n / 2 + rad * np.cos(ang)
     vy = n / 2 + rad * np.sin(ang)
     y, x = np.mgrid[0:n, 0:n].astype(np.float64)
@@ -148,3 +152,22 @@ def random_start(n, P, support, seed=0, scale=1.0):
     rng = np.random.default_rng(seed)
     rho = (rng.random((P, n, n)) * np.exp(1j * rng.uniform(-0.2, 0.2, (P, n, n)))) * support
     return (rho * scale).astype(np.complex64)
+
+
+BATCH_SIZES = (256, 384, 512, 768, 1024)
+
+
+def batch_sizes(count, seed=0):
+    """Sizes of `count` batch patterns, drawn uniformly from BATCH_SIZES."""
+    rng = np.random.default_rng(seed)
+    return [int(BATCH_SIZES[i]) for i in rng.integers(0, len(BATCH_SIZES), count)]
+
+
+def batch_patterns(count, seed=0):
+    """`count` independent polygon patterns for a mixed-size batch: sizes from
+    batch_sizes(count, seed), objects as polygon_pattern scaled by n/128,
+    photons log-uniform in [1e7, 1e10]. Yields (intensity, rho_true) lazily."""
+    rng = np.random.default_rng(seed + 1)
+    photons = 10.0 ** rng.uniform(7, 10, count)
+    for i, n in enumerate(batch_sizes(count, seed)):
+        yield polygon_pattern(seed=seed * 100_003 + i, n=n, photons=float(photons[i]), scale=n / 128)

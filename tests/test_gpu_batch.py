@@ -1,0 +1,83 @@
+"""Batches of patterns of several sizes (spr_batch) through the C ABI.
+
+Every individual's arithmetic in a batch is the arithmetic of the single-pattern
+handle (the same kernels, reading the individual's own amplitude), and every
+reduction has a fixed order, so the bar is bit-identity with one single-pattern
+handle per pattern, whose parity with the oracle the other GPU tests establish.
+A block replayed from its HIP graph must equal the same launches issued
+directly, on every replay, with ordinary work and synchronisations between
+replays.
+"""
+import numpy as np
+import pytest
+
+from helpers import case
+from s_spi_spring_amd import hipcore
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [128, 384, 128, 256, 768, 384]
+POPS = [3, 2, 2, 2, 1, 2]
+SEQ = [("HIO", [5, 0.9]), ("SW", [2.0, 0.1]), ("RAAR", [3, 0.87]), ("SWA", [2.0, 900]), ("ER", [2])]
+
+
+def cases():
+    return [case(n, P, seed=20 + i) for i, (n, P) in enumerate(zip(SIZES, POPS))]
+
+
+def test_batch_equals_single_pattern_handles(hip):
+    cs = cases()
+    print("batch: sizes", SIZES, "populations", POPS, flush=True)
+    b = hipcore.Batch([c[0] for c in cs], POPS)
+    for k, (amp, rho, supp) in enumerate(cs):
+        b.set_population(k, rho, supp)
+    b.run(SEQ, nrep=2, reality=0.0)
+    ev = b.evaluate(0.0)
+    for k, (amp, rho, supp) in enumerate(cs):
+        one = hip.Backend(amp, capacity=len(rho))
+        one.set_population(rho, supp)
+        one.run(SEQ, nrep=2, reality=0.0)
+        want_rho, want_s = one.get_population()
+        got_rho, got_s = b.get_population(k)
+        assert np.array_equal(got_s, want_s), k
+        assert np.array_equal(got_rho, want_rho), k
+        e1 = one.evaluate(0.0, symmetry=False)
+        for name in ("error_fourier", "error_real", "complexsum", "modulesum"):
+            assert np.array_equal(ev[name][k], e1[name]), (k, name)
+        one.close()
+    b.close()
+
+
+def test_graph_replays_equal_direct_launches(hip):
+    cs = cases()
+    a = hipcore.Batch([c[0] for c in cs], POPS)
+    d = hipcore.Batch([c[0] for c in cs], POPS)
+    d.graphs(False)
+    for k, (amp, rho, supp) in enumerate(cs):
+        a.set_population(k, rho, supp)
+        d.set_population(k, rho, supp)
+    for rep in range(3):      # the first call captures, the others replay
+        a.run([("RAAR", [4, 0.87])])
+        d.run([("RAAR", [4, 0.87])])
+        ea, ed = a.evaluate(0.0), d.evaluate(0.0)
+        for k in range(len(SIZES)):
+            assert np.array_equal(a.get_population(k, support=False), d.get_population(k, support=False)), (rep, k)
+            assert np.array_equal(ea["error_fourier"][k], ed["error_fourier"][k]), (rep, k)
+        a.run([("SW", [2.5, 0.12])])      # eager work between replays
+        d.run([("SW", [2.5, 0.12])])
+    a.close()
+    d.close()
+
+
+def test_batch_argument_errors(hip):
+    amp128 = case(128, 1)[0]
+    with pytest.raises(hipcore.SpringHipError, match="unsupported grid size"):
+        hipcore.Batch([amp128, np.zeros((100, 100), np.float32)], [2, 2])
+    with pytest.raises(hipcore.SpringHipError, match="empty population"):
+        hipcore.Batch([amp128], [0])
+    b = hipcore.Batch([amp128], [2])
+    assert b.lib.spr_batch_get_population(b.h, 1, None, None) != 0
+    assert b"out of range" in b.lib.spr_batch_last_error(b.h)
+    with pytest.raises(hipcore.SpringHipError, match="negative iteration"):
+        b.run([("HIO", [-1, 0.9])])
+    b.close()
