@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "individual-iters/s at 512² pop64; time-to-Rf<0.05, batched+grad vs plain GPU"
 ROW_MODE = {"HIO": 2, "RAAR": 3, "ER": 4}       # RowMode of spr_kernels.hpp
 TIMING_STRIDE = 8
-DEVICE_SOURCES = ("spr_fft.hpp", "spr_kernels.hpp", "spr_aux.hpp", "spr_grad.hpp", "spr_backend.hip")
+DEVICE_SOURCES = ("spr_fft.hpp", "spr_kernels.hpp", "spr_aux.hpp", "spr_grad.hpp", "spr_tuned.hpp", "spr_backend.hip")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_iteration_512_pop64_raar.json")
 COUNTER_CMD = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 90 rocprofv3 --pmc {ctr} --kernel-trace "
                "--output-format csv -d <dir> -o run -- python3 -u /root/repo/bench.py --steps 2 --warmup 1 "
@@ -107,10 +107,10 @@ def device_source_hash():
     return h.hexdigest()[:16]
 
 
-def rocprof_kernel_name(kernel, n, alg):
+def rocprof_kernel_name(kernel, n, alg, variant):
     if kernel == "row":
-        return f"void spr::k_row<{n}, {ROW_MODE[alg]}>(spr::RowArgs)"
-    return f"void spr::k_col<{n}, 0>(spr::ColArgs)"
+        return f"void spr::k_row<{n}, {ROW_MODE[alg]}, {variant}>(spr::RowArgs)"
+    return f"void spr::k_col<{n}, 0, {variant}>(spr::ColArgs)"
 
 
 def _counter_values(run_dir, counter):
@@ -373,11 +373,12 @@ def main():
         col_gbs = col_b / col_avg / 1e9
         row_gbs = row_b / row_avg / 1e9
         # dominant kernel = the one with the larger share of the timed region
+        cvar, rvar = be.variants()
         if row_ms >= col_ms:
-            dom = {"kernel": rocprof_kernel_name("row", n, args.alg), "achieved": row_gbs,
+            dom = {"kernel": rocprof_kernel_name("row", n, args.alg, rvar), "achieved": row_gbs,
                    "bytes_per_launch": row_b, "avg_launch_us": row_avg * 1e6}
         else:
-            dom = {"kernel": rocprof_kernel_name("col", n, args.alg), "achieved": col_gbs,
+            dom = {"kernel": rocprof_kernel_name("col", n, args.alg, cvar), "achieved": col_gbs,
                    "bytes_per_launch": col_b, "avg_launch_us": col_avg * 1e6}
         log(f"column pass: {col_cnt} launches, avg {col_avg * 1e6:.1f} us, {col_gbs:.0f} GB/s algorithmic")
         log(f"row pass:    {row_cnt} launches, avg {row_avg * 1e6:.1f} us, {row_gbs:.0f} GB/s algorithmic")
@@ -407,7 +408,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"ellipsoids {n}^2 pattern, population {P}, {args.alg}(beta={args.beta}) "
                                f"blocks of {args.block} iterations, population resident in HBM",
-                   "n": n, "population": P, "alg": args.alg, "iterations_per_step": args.block},
+                   "n": n, "population": P, "alg": args.alg, "iterations_per_step": args.block,
+                   "kernel_variants": dict(zip(("column", "row"), be.variants()))},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

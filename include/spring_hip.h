@@ -261,6 +261,37 @@ int spr_timing_read(spr_ctx* ctx, int kind, double* total_ms, long* count);
 int spr_stream(spr_ctx* ctx, void** stream);
 
 /* ---------------------------------------------------------------------------
+ * Kernel variants and the autotuner. No counterpart in the reference, whose
+ * transforms are cuFFT plans (src/gpu/gpu_context.h:29-32).
+ *
+ * The column pass (colFFT -> P_M -> colIFFT) and the row pass (rowIFFT ->
+ * update -> rowFFT) of each grid size are compiled in several variants:
+ * columns per workgroup, values per thread (radix plan), threads and prefetch
+ * of the row pass. Variant 0 is the default geometry; all variants read and
+ * write the same layout. Width, threads and prefetch run the same operations
+ * (up to the compiler's contraction of multiply-adds); another radix plan
+ * changes the rounding of the transforms. Every variant meets the same
+ * accuracy bar against the oracle.
+ *
+ * spr_variants: number of column and row variants of size n.
+ * spr_autotune: times `reps` launches of every variant on the handle's
+ *   resident population (densities and supports are left unchanged), keeps
+ *   the fastest pair for this handle and makes it the default of handles of
+ *   that size created later in the process; col_ms / row_ms (NULL or one
+ *   entry per variant) receive the mean launch durations in ms.
+ * spr_set_variants / spr_get_variants: one handle's pair.
+ * spr_default_variants / spr_set_default_variants: the pair new handles of
+ *   size n start with (initially the table compiled from the last autotuning
+ *   run on the target GPU, spr_tuned.hpp).
+ * spr_batch_autotune: spr_autotune on every size class of a batch. */
+int spr_variants(int n, int* col_count, int* row_count);
+int spr_autotune(spr_ctx* ctx, int reps, float* col_ms, float* row_ms);
+int spr_set_variants(spr_ctx* ctx, int col, int row);
+int spr_get_variants(spr_ctx* ctx, int* col, int* row);
+int spr_default_variants(int n, int* col, int* row);
+int spr_set_default_variants(int n, int col, int row);
+
+/* ---------------------------------------------------------------------------
  * Batches of independent patterns of several sizes on one device.
  *
  * No counterpart in the reference, which reconstructs one pattern per Solver
@@ -290,6 +321,7 @@ int spr_batch_get_population(spr_batch* b, int pattern, float* rho, int32_t* sup
 int spr_batch_run(spr_batch* b, const spr_op* ops, int nops, int nrep, float reality);
 int spr_batch_evaluate(spr_batch* b, float reality, float* error_fourier, float* error_real, float* complexsum,
                        float* modulesum);
+int spr_batch_autotune(spr_batch* b, int reps);
 
 #ifdef __cplusplus
 }

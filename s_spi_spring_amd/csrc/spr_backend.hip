@@ -8,12 +8,14 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
 
 #include "spring_hip.h"
 #include "spr_grad.hpp"
+#include "spr_tuned.hpp"
 
 using namespace spr;
 
@@ -35,6 +37,7 @@ struct spr_ctx {
     float* amp = nullptr;         // amplitudes [npat][n][n]
     float* amp_t = nullptr;       // amp transposed (read along columns by the column pass)
     int npat = 1;                 // patterns sharing this handle (one amplitude each)
+    int colvar = 0, rowvar = 0;   // variants of the column and row iteration passes
     int* amp_of = nullptr;        // slot -> pattern when npat > 1, else NULL
     float2* tw = nullptr;
     // reduction scratch
@@ -144,34 +147,72 @@ PhaseCut cut_raar(float r) {
 }
 
 // Row / column passes over `cnt` individuals starting at the pointers in the
-// arguments, on stream `s`.
-template <int N> int launch_row(spr_ctx* c, int mode, const RowArgs& a, int cnt, hipStream_t s) {
-    const dim3 grid((unsigned)((size_t)cnt * N / Geo<N>::RW)), block(Geo<N>::ROW_THREADS);
+// arguments, on stream `s`, in the handle's variant.
+template <int N, int RV> int launch_row_v(spr_ctx* c, int mode, const RowArgs& a, int cnt, hipStream_t s) {
+    using G = typename RowVariant<N, RV>::G;
+    const dim3 grid((unsigned)((size_t)cnt * N / G::RW)), block(G::ROW_THREADS);
     switch (mode) {
-        case ROW_FWD: hipLaunchKernelGGL((k_row<N, ROW_FWD>), grid, block, 0, s, a); break;
-        case ROW_FINAL: hipLaunchKernelGGL((k_row<N, ROW_FINAL>), grid, block, 0, s, a); break;
-        case ROW_HIO: hipLaunchKernelGGL((k_row<N, ROW_HIO>), grid, block, 0, s, a); break;
-        case ROW_RAAR: hipLaunchKernelGGL((k_row<N, ROW_RAAR>), grid, block, 0, s, a); break;
-        case ROW_ER: hipLaunchKernelGGL((k_row<N, ROW_ER>), grid, block, 0, s, a); break;
-        case ROW_FWD_INPLACE: hipLaunchKernelGGL((k_row<N, ROW_FWD_INPLACE>), grid, block, 0, s, a); break;
-        case ROW_INV_INPLACE: hipLaunchKernelGGL((k_row<N, ROW_INV_INPLACE>), grid, block, 0, s, a); break;
+        case ROW_FWD: hipLaunchKernelGGL((k_row<N, ROW_FWD, RV>), grid, block, 0, s, a); break;
+        case ROW_FINAL: hipLaunchKernelGGL((k_row<N, ROW_FINAL, RV>), grid, block, 0, s, a); break;
+        case ROW_HIO: hipLaunchKernelGGL((k_row<N, ROW_HIO, RV>), grid, block, 0, s, a); break;
+        case ROW_RAAR: hipLaunchKernelGGL((k_row<N, ROW_RAAR, RV>), grid, block, 0, s, a); break;
+        case ROW_ER: hipLaunchKernelGGL((k_row<N, ROW_ER, RV>), grid, block, 0, s, a); break;
+        case ROW_FWD_INPLACE: hipLaunchKernelGGL((k_row<N, ROW_FWD_INPLACE, RV>), grid, block, 0, s, a); break;
+        case ROW_INV_INPLACE: hipLaunchKernelGGL((k_row<N, ROW_INV_INPLACE, RV>), grid, block, 0, s, a); break;
         default: return fail(c, SPR_ERR_ARG, "bad row mode");
     }
     SPR_HIP(c, hipGetLastError());
     return SPR_OK;
 }
+template <int N> int launch_row(spr_ctx* c, int mode, const RowArgs& a, int cnt, hipStream_t s) {
+    switch (c->rowvar) {
+        case 0: return launch_row_v<N, 0>(c, mode, a, cnt, s);
+        case 1:
+            if constexpr (RowVariant<N, 1>::valid) return launch_row_v<N, 1>(c, mode, a, cnt, s);
+            break;
+        case 2:
+            if constexpr (RowVariant<N, 2>::valid) return launch_row_v<N, 2>(c, mode, a, cnt, s);
+            break;
+        case 3:
+            if constexpr (RowVariant<N, 3>::valid) return launch_row_v<N, 3>(c, mode, a, cnt, s);
+            break;
+        case 4:
+            if constexpr (RowVariant<N, 4>::valid) return launch_row_v<N, 4>(c, mode, a, cnt, s);
+            break;
+    }
+    return fail(c, SPR_ERR_ARG, "bad row variant");
+}
 
-template <int N> int launch_col(spr_ctx* c, int mode, float2* f, int cnt, hipStream_t s) {
+template <int N, int CV> int launch_col_v(spr_ctx* c, int mode, float2* f, int cnt, hipStream_t s) {
+    using G = typename ColVariant<N, CV>::G;
     const ColArgs a{f, c->amp, c->tw, c->amp_t, c->amp_of};
-    const dim3 grid((unsigned)((size_t)cnt * (N / Geo<N>::C))), block(Geo<N>::COL_THREADS);
+    const dim3 grid((unsigned)((size_t)cnt * (N / G::C))), block(G::COL_THREADS);
     switch (mode) {
-        case COL_PROJ: hipLaunchKernelGGL((k_col<N, COL_PROJ>), grid, block, 0, s, a); break;
-        case COL_FWD: hipLaunchKernelGGL((k_col<N, COL_FWD>), grid, block, 0, s, a); break;
-        case COL_INV: hipLaunchKernelGGL((k_col<N, COL_INV>), grid, block, 0, s, a); break;
+        case COL_PROJ: hipLaunchKernelGGL((k_col<N, COL_PROJ, CV>), grid, block, 0, s, a); break;
+        case COL_FWD: hipLaunchKernelGGL((k_col<N, COL_FWD, CV>), grid, block, 0, s, a); break;
+        case COL_INV: hipLaunchKernelGGL((k_col<N, COL_INV, CV>), grid, block, 0, s, a); break;
         default: return fail(c, SPR_ERR_ARG, "bad column mode");
     }
     SPR_HIP(c, hipGetLastError());
     return SPR_OK;
+}
+template <int N> int launch_col(spr_ctx* c, int mode, float2* f, int cnt, hipStream_t s) {
+    switch (c->colvar) {
+        case 0: return launch_col_v<N, 0>(c, mode, f, cnt, s);
+        case 1:
+            if constexpr (ColVariant<N, 1>::valid) return launch_col_v<N, 1>(c, mode, f, cnt, s);
+            break;
+        case 2:
+            if constexpr (ColVariant<N, 2>::valid) return launch_col_v<N, 2>(c, mode, f, cnt, s);
+            break;
+        case 3:
+            if constexpr (ColVariant<N, 3>::valid) return launch_col_v<N, 3>(c, mode, f, cnt, s);
+            break;
+        case 4:
+            if constexpr (ColVariant<N, 4>::valid) return launch_col_v<N, 4>(c, mode, f, cnt, s);
+            break;
+    }
+    return fail(c, SPR_ERR_ARG, "bad column variant");
 }
 
 // Grid sizes with compiled kernels (RadixPlan in spr_fft.hpp).
@@ -699,6 +740,123 @@ int final_loss(spr_ctx* c, int kind, float eps, double* loss) {
     return SPR_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Variant choice and the autotuner
+// ---------------------------------------------------------------------------
+struct VariantPair {
+    int col = 0, row = 0;
+};
+std::mutex g_variant_mu;
+std::map<int, VariantPair> g_variant;   // sizes whose default differs from the compiled table
+
+int variant_counts(int n, int* nc, int* nr) {
+#define SPR_VAR_CASE(N_)                      \
+    if (n == N_) {                            \
+        *nc = col_variants<N_>();             \
+        *nr = row_variants<N_>();             \
+        return SPR_OK;                        \
+    }
+    SPR_SIZES(SPR_VAR_CASE)
+#undef SPR_VAR_CASE
+    return SPR_ERR_ARG;
+}
+
+// Variants a new handle of size n starts with: the last spr_autotune or
+// spr_set_default_variants of this process for n, else the compiled table.
+VariantPair default_variants(int n) {
+    std::lock_guard<std::mutex> lk(g_variant_mu);
+    auto it = g_variant.find(n);
+    if (it != g_variant.end()) return it->second;
+    VariantPair v;
+    for (const TunedVariants& t : kTuned)
+        if (t.n == n) {
+            v.col = t.col;
+            v.row = t.row;
+        }
+    int nc = 1, nr = 1;
+    if (variant_counts(n, &nc, &nr) != SPR_OK || v.col >= nc || v.row >= nr) v = VariantPair{};
+    return v;
+}
+
+// Mean duration (ms) of `reps` launches of one pass in the handle's current variant.
+int time_pass(spr_ctx* c, bool column, const RowArgs& ra, int reps, hipEvent_t e0, hipEvent_t e1, float* ms) {
+    auto once = [&]() { return column ? col_n(c, COL_PROJ, c->work, c->P, c->stream) : row_n(c, ROW_RAAR, ra, c->P, c->stream); };
+    SPR_TRY(once());   // warm-up (code load, caches)
+    SPR_HIP(c, hipEventRecord(e0, c->stream));
+    for (int i = 0; i < reps; ++i) SPR_TRY(once());
+    SPR_HIP(c, hipEventRecord(e1, c->stream));
+    SPR_HIP(c, hipEventSynchronize(e1));
+    float t = 0.f;
+    SPR_HIP(c, hipEventElapsedTime(&t, e0, e1));
+    *ms = t / (float)reps;
+    return SPR_OK;
+}
+
+// Times every column and row variant of the handle's size on its resident
+// population and keeps the fastest. The densities and supports are not
+// changed: the row variants update a copy of the densities in work2 and the
+// column variants transform `work`, which every operation rewrites first.
+int autotune(spr_ctx* c, int reps, float* col_ms, float* row_ms) {
+    if (c->P <= 0) return fail(c, SPR_ERR_ARG, "autotune needs resident individuals");
+    if (reps < 1) return fail(c, SPR_ERR_ARG, "reps must be >= 1");
+    int nc = 1, nr = 1;
+    SPR_TRY(variant_counts(c->n, &nc, &nr));
+    const int np = c->n * c->n;
+    SPR_HIP(c, hipSetDevice(c->device));
+    hipLaunchKernelGGL(k_copy_fields, dim3((np + 255) / 256, c->P), dim3(256), 0, c->stream, (const float2*)c->rho,
+                       (const int*)nullptr, c->work2, (const int*)nullptr, np);
+    SPR_HIP(c, hipGetLastError());
+    RowArgs ra = row_args(c);
+    ra.rho = c->work2;
+    ra.beta = 0.87f;
+    ra.pc = cut_raar(0.f);
+    const VariantPair keep{c->colvar, c->rowvar};
+    c->rowvar = 0;
+    SPR_TRY(row_n(c, ROW_FWD, ra, c->P, c->stream));   // realistic spectra in work
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SPR_HIP(c, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(c, SPR_ERR_HIP, "hipEventCreate failed");
+    }
+    VariantPair best;
+    float best_c = 0.f, best_r = 0.f;
+    int rc = SPR_OK;
+    for (int v = 0; v < nc && rc == SPR_OK; ++v) {
+        float ms = 0.f;
+        c->colvar = v;
+        rc = time_pass(c, true, ra, reps, e0, e1, &ms);
+        if (col_ms) col_ms[v] = ms;
+        if (rc == SPR_OK && (v == 0 || ms < best_c)) {
+            best_c = ms;
+            best.col = v;
+        }
+    }
+    c->colvar = keep.col;
+    for (int v = 0; v < nr && rc == SPR_OK; ++v) {
+        float ms = 0.f;
+        c->rowvar = v;
+        rc = time_pass(c, false, ra, reps, e0, e1, &ms);
+        if (row_ms) row_ms[v] = ms;
+        if (rc == SPR_OK && (v == 0 || ms < best_r)) {
+            best_r = ms;
+            best.row = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != SPR_OK) {
+        c->colvar = keep.col;
+        c->rowvar = keep.row;
+        return rc;
+    }
+    c->colvar = best.col;
+    c->rowvar = best.row;
+    std::lock_guard<std::mutex> lk(g_variant_mu);
+    g_variant[c->n] = best;
+    return SPR_OK;
+}
+
 int check_indices(spr_ctx* c, const int32_t* idx, int count, int limit, const char* what) {
     for (int i = 0; i < count; ++i)
         if (idx[i] < 0 || idx[i] >= limit) return fail(c, SPR_ERR_ARG, std::string(what) + ": index out of range");
@@ -741,6 +899,11 @@ static int create_context(spr_ctx** out, int n, int capacity, int npat, const fl
     c->cap = capacity;
     c->device = device;
     c->npat = npat;
+    {
+        const VariantPair v = default_variants(n);
+        c->colvar = v.col;
+        c->rowvar = v.row;
+    }
     const size_t np = (size_t)n * n;
     auto cleanup_fail = [&](const char* what, hipError_t e) {
         g_create_error = std::string(what) + ": " + hipGetErrorString(e);
@@ -1553,6 +1716,57 @@ int spr_stream(spr_ctx* c, void** stream) {
     return SPR_OK;
 }
 
+int spr_variants(int n, int* col_count, int* row_count) {
+    int nc = 0, nr = 0;
+    if (variant_counts(n, &nc, &nr) != SPR_OK) return fail(nullptr, SPR_ERR_ARG, "unsupported grid size");
+    if (col_count) *col_count = nc;
+    if (row_count) *row_count = nr;
+    return SPR_OK;
+}
+
+int spr_default_variants(int n, int* col, int* row) {
+    int nc = 0, nr = 0;
+    if (variant_counts(n, &nc, &nr) != SPR_OK) return fail(nullptr, SPR_ERR_ARG, "unsupported grid size");
+    const VariantPair v = default_variants(n);
+    if (col) *col = v.col;
+    if (row) *row = v.row;
+    return SPR_OK;
+}
+
+int spr_set_default_variants(int n, int col, int row) {
+    int nc = 0, nr = 0;
+    if (variant_counts(n, &nc, &nr) != SPR_OK) return fail(nullptr, SPR_ERR_ARG, "unsupported grid size");
+    if (col < 0 || col >= nc || row < 0 || row >= nr) return fail(nullptr, SPR_ERR_ARG, "variant out of range");
+    std::lock_guard<std::mutex> lk(g_variant_mu);
+    g_variant[n] = VariantPair{col, row};
+    return SPR_OK;
+}
+
+int spr_set_variants(spr_ctx* c, int col, int row) {
+    if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
+    int nc = 0, nr = 0;
+    SPR_TRY(variant_counts(c->n, &nc, &nr));
+    if (col < 0 || col >= nc || row < 0 || row >= nr) return fail(c, SPR_ERR_ARG, "variant out of range");
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    c->colvar = col;
+    c->rowvar = row;
+    return SPR_OK;
+}
+
+int spr_get_variants(spr_ctx* c, int* col, int* row) {
+    if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
+    if (col) *col = c->colvar;
+    if (row) *row = c->rowvar;
+    return SPR_OK;
+}
+
+int spr_autotune(spr_ctx* c, int reps, float* col_ms, float* row_ms) {
+    if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
+    SPR_HIP(c, hipSetDevice(c->device));
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    return autotune(c, reps, col_ms, row_ms);
+}
+
 // ---------------------------------------------------------------------------
 // Batches of patterns of several sizes (spring_hip.h, spr_batch_*)
 // ---------------------------------------------------------------------------
@@ -1822,6 +2036,16 @@ int spr_batch_run(spr_batch* b, const spr_op* ops, int nops, int nrep, float rea
             }
         }
     }
+    return batch_sync(b);
+}
+
+int spr_batch_autotune(spr_batch* b, int reps) {
+    if (!b) return bfail(b, SPR_ERR_ARG, "NULL handle");
+    SPR_TRY(batch_sync(b));
+    for (auto& k : b->classes) SPR_TRY(bclass(b, k, autotune(k.ctx, reps, nullptr, nullptr)));
+    // captured blocks hold the previous variants' kernels
+    for (auto& kv : b->blocks) (void)hipGraphExecDestroy(kv.second);
+    b->blocks.clear();
     return batch_sync(b);
 }
 

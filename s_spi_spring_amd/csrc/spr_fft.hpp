@@ -144,8 +144,61 @@ template <int S> __device__ __forceinline__ void dft12(float2* v) {
     }
 }
 
+// z * exp(S*2*pi*i*M/16), M a compile-time multiple of 22.5 degrees.
+template <int M, int S> __device__ __forceinline__ float2 rot16(float2 z) {
+    constexpr int m = M % 16;
+    if constexpr (m == 0) {
+        return z;
+    } else if constexpr (m == 4) {
+        return mul_si<S>(z);
+    } else if constexpr (m == 8) {
+        return make_float2(-z.x, -z.y);
+    } else if constexpr (m == 12) {
+        return mul_si<-S>(z);
+    } else {
+        static_assert(m == 1 || m == 2 || m == 3 || m == 6 || m == 9, "twiddle");
+        constexpr float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f, h = 0.70710678118654752440f;
+        // cos and sin of m*22.5 degrees
+        constexpr float c = m == 1 ? c1 : m == 2 ? h : m == 3 ? s1 : m == 6 ? -h : -c1;
+        constexpr float s = m == 1 ? s1 : m == 2 ? h : m == 3 ? c1 : m == 6 ? h : -s1;
+        return cmul(z, make_float2(c, S * s));
+    }
+}
+
+// 16-point DFT as 4 x 4 (Cooley-Tukey, n = n1 + 4*n2, k = k2 + 4*k1).
+template <int S> __device__ __forceinline__ void dft16(float2* v) {
+    float2 y[4][4];
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) {
+        y[n1][0] = v[n1];
+        y[n1][1] = v[n1 + 4];
+        y[n1][2] = v[n1 + 8];
+        y[n1][3] = v[n1 + 12];
+        dft4<S>(y[n1][0], y[n1][1], y[n1][2], y[n1][3]);
+    }
+    y[1][1] = rot16<1, S>(y[1][1]);
+    y[1][2] = rot16<2, S>(y[1][2]);
+    y[1][3] = rot16<3, S>(y[1][3]);
+    y[2][1] = rot16<2, S>(y[2][1]);
+    y[2][2] = rot16<4, S>(y[2][2]);
+    y[2][3] = rot16<6, S>(y[2][3]);
+    y[3][1] = rot16<3, S>(y[3][1]);
+    y[3][2] = rot16<6, S>(y[3][2]);
+    y[3][3] = rot16<9, S>(y[3][3]);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) {
+        float2 a0 = y[0][k2], a1 = y[1][k2], a2 = y[2][k2], a3 = y[3][k2];
+        dft4<S>(a0, a1, a2, a3);
+        v[k2] = a0;
+        v[k2 + 4] = a1;
+        v[k2 + 8] = a2;
+        v[k2 + 12] = a3;
+    }
+}
+
 template <int R, int S> __device__ __forceinline__ void dftR(float2* v) {
-    if constexpr (R == 12) dft12<S>(v);
+    if constexpr (R == 16) dft16<S>(v);
+    else if constexpr (R == 12) dft12<S>(v);
     else if constexpr (R == 8) dft8<S>(v);
     else if constexpr (R == 6) dft6<S>(v);
     else if constexpr (R == 4) dft4<S>(v[0], v[1], v[2], v[3]);
@@ -153,77 +206,89 @@ template <int R, int S> __device__ __forceinline__ void dftR(float2* v) {
     else dft2<S>(v[0], v[1]);
 }
 
-// Radix plans: V values per thread, ns stages with radices r[] (each dividing V).
-template <int N> struct RadixPlan;
-template <> struct RadixPlan<64>   { static constexpr int V = 8;  static constexpr int ns = 2; static constexpr int r[4] = {8, 8, 0, 0}; };
-template <> struct RadixPlan<128>  { static constexpr int V = 8;  static constexpr int ns = 3; static constexpr int r[4] = {8, 2, 8, 0}; };
-template <> struct RadixPlan<256>  { static constexpr int V = 8;  static constexpr int ns = 3; static constexpr int r[4] = {8, 4, 8, 0}; };
-template <> struct RadixPlan<384>  { static constexpr int V = 12; static constexpr int ns = 4; static constexpr int r[4] = {12, 4, 4, 2}; };
-template <> struct RadixPlan<512>  { static constexpr int V = 8;  static constexpr int ns = 3; static constexpr int r[4] = {8, 8, 8, 0}; };
-template <> struct RadixPlan<768>  { static constexpr int V = 12; static constexpr int ns = 4; static constexpr int r[4] = {12, 4, 4, 4}; };
-template <> struct RadixPlan<1024> { static constexpr int V = 8;  static constexpr int ns = 4; static constexpr int r[4] = {8, 2, 8, 8}; };
-template <> struct RadixPlan<2048> { static constexpr int V = 8;  static constexpr int ns = 4; static constexpr int r[4] = {8, 4, 8, 8}; };
+// A radix plan: the N-point transform with V values per thread in ns stages of
+// radices r[] (each dividing V; unused entries 0).
+template <int N_, int V_, int R0, int R1, int R2 = 0, int R3 = 0> struct Plan {
+    static constexpr int N = N_;
+    static constexpr int V = V_;
+    static constexpr int ns = R2 == 0 ? 2 : R3 == 0 ? 3 : 4;
+    static constexpr int r[4] = {R0, R1, R2, R3};
+    static_assert(R0 * R1 * (R2 ? R2 : 1) * (R3 ? R3 : 1) == N_, "radices must multiply to N");
+    static_assert(V_ % R0 == 0 && V_ % R1 == 0 && (R2 == 0 || V_ % R2 == 0) && (R3 == 0 || V_ % R3 == 0),
+                  "every radix must divide the values per thread");
+};
 
-template <int N> constexpr int span_before(int s) {
+// Default plan of each supported size.
+template <int N> struct RadixPlan;
+template <> struct RadixPlan<64> : Plan<64, 8, 8, 8> {};
+template <> struct RadixPlan<128> : Plan<128, 8, 8, 2, 8> {};
+template <> struct RadixPlan<256> : Plan<256, 8, 8, 4, 8> {};
+template <> struct RadixPlan<384> : Plan<384, 12, 12, 4, 4, 2> {};
+template <> struct RadixPlan<512> : Plan<512, 8, 8, 8, 8> {};
+template <> struct RadixPlan<768> : Plan<768, 12, 12, 4, 4, 4> {};
+template <> struct RadixPlan<1024> : Plan<1024, 8, 8, 2, 8, 8> {};
+template <> struct RadixPlan<2048> : Plan<2048, 8, 8, 4, 8, 8> {};
+
+template <class PL> constexpr int span_before(int s) {
     int ns = 1;
-    for (int i = 0; i < s; ++i) ns *= RadixPlan<N>::r[i];
+    for (int i = 0; i < s; ++i) ns *= PL::r[i];
     return ns;
 }
 
 // Per-stage twiddle tables in LDS. Stage s >= 1 (radix R, span NS) needs
 // w^(r*k) = root[r*k*N/(NS*R)] for k < NS, 1 <= r < R; they are stored densely
-// at tws[off(s) + k*(R-1) + r-1]. The row stride R-1 is odd (R in {2, 4, 8, 12}),
+// at tws[off(s) + k*(R-1) + r-1]. The row stride R-1 is odd (R in {2, 4, 8, 12, 16}),
 // so the 8-byte reads of 16 consecutive k fall in distinct LDS bank pairs; one
 // base address serves all r.
-template <int N> struct TwTable {
+template <class PL> struct TwTableP {
     static constexpr int off(int s) {
         int o = 0;
-        for (int i = 1; i < s; ++i) o += span_before<N>(i) * (RadixPlan<N>::r[i] - 1);
+        for (int i = 1; i < s; ++i) o += span_before<PL>(i) * (PL::r[i] - 1);
         return o;
     }
-    static constexpr int SIZE = off(RadixPlan<N>::ns);
+    static constexpr int SIZE = off(PL::ns);
 };
+template <int N> using TwTable = TwTableP<RadixPlan<N>>;
 
-template <int N, int S>
+template <class PL, int S>
 __device__ __forceinline__ void tw_fill_stage(float2* tws, const float2* __restrict__ root, int tid, int nt) {
-    if constexpr (S < RadixPlan<N>::ns) {
-        constexpr int R = RadixPlan<N>::r[S];
-        constexpr int NS = span_before<N>(S);
-        constexpr int STEP = N / (NS * R);
-        constexpr int OFF = TwTable<N>::off(S);
+    if constexpr (S < PL::ns) {
+        constexpr int R = PL::r[S];
+        constexpr int NS = span_before<PL>(S);
+        constexpr int STEP = PL::N / (NS * R);
+        constexpr int OFF = TwTableP<PL>::off(S);
         for (int e = tid; e < NS * (R - 1); e += nt) {
             const int k = e / (R - 1), r = e % (R - 1) + 1;
             tws[OFF + k * (R - 1) + r - 1] = root[r * k * STEP];
         }
-        tw_fill_stage<N, S + 1>(tws, root, tid, nt);
+        tw_fill_stage<PL, S + 1>(tws, root, tid, nt);
     }
 }
 
 // Fill the per-stage tables from the global root table exp(-2 pi i m/N) (the
 // caller synchronises before the first transform).
+template <class PL>
+__device__ __forceinline__ void tw_fill_p(float2* tws, const float2* __restrict__ root, int tid, int nt) {
+    tw_fill_stage<PL, 1>(tws, root, tid, nt);
+}
 template <int N>
 __device__ __forceinline__ void tw_fill(float2* tws, const float2* __restrict__ root, int tid, int nt) {
-    tw_fill_stage<N, 1>(tws, root, tid, nt);
+    tw_fill_p<RadixPlan<N>>(tws, root, tid, nt);
 }
 
-// Values per thread and threads per line of the N-point transform.
-template <int N> constexpr int line_values() { return RadixPlan<N>::V; }
-template <int N> constexpr int line_threads() { return N / RadixPlan<N>::V; }
-
 // Twiddle and butterfly of stage STAGE on registers v (V values = V/R butterflies).
-template <int N, int STAGE, int SIGN>
-__device__ __forceinline__ void stage_compute(float2 (&v)[RadixPlan<N>::V], const float2* __restrict__ tws, int t) {
-    constexpr int R = RadixPlan<N>::r[STAGE];
-    constexpr int NS = span_before<N>(STAGE);
-    constexpr int T = line_threads<N>();
-    constexpr int NB = RadixPlan<N>::V / R;
-    static_assert(NB * R == RadixPlan<N>::V, "radix must divide the values per thread");
+template <class PL, int STAGE, int SIGN>
+__device__ __forceinline__ void stage_compute(float2 (&v)[PL::V], const float2* __restrict__ tws, int t) {
+    constexpr int R = PL::r[STAGE];
+    constexpr int NS = span_before<PL>(STAGE);
+    constexpr int T = PL::N / PL::V;
+    constexpr int NB = PL::V / R;
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
         if constexpr (NS > 1) {
             const int b = t + T * u;
             const int k = b % NS;
-            const float2* w = tws + TwTable<N>::off(STAGE) + k * (R - 1);
+            const float2* w = tws + TwTableP<PL>::off(STAGE) + k * (R - 1);
 #pragma unroll
             for (int r = 1; r < R; ++r) {
                 float2 wr = w[r - 1];
@@ -236,12 +301,12 @@ __device__ __forceinline__ void stage_compute(float2 (&v)[RadixPlan<N>::V], cons
 }
 
 // Stage inputs of stage STAGE from LDS: v[u*R + r] = x[b + r*N/R], b = t + T*u.
-template <int N, int STAGE, class Acc>
-__device__ __forceinline__ void stage_load(float2 (&v)[RadixPlan<N>::V], const Acc& acc, int t) {
-    constexpr int R = RadixPlan<N>::r[STAGE];
-    constexpr int T = line_threads<N>();
-    constexpr int NB = RadixPlan<N>::V / R;
-    constexpr int M = N / R;
+template <class PL, int STAGE, class Acc>
+__device__ __forceinline__ void stage_load(float2 (&v)[PL::V], const Acc& acc, int t) {
+    constexpr int R = PL::r[STAGE];
+    constexpr int T = PL::N / PL::V;
+    constexpr int NB = PL::V / R;
+    constexpr int M = PL::N / R;
 #pragma unroll
     for (int u = 0; u < NB; ++u)
 #pragma unroll
@@ -249,12 +314,12 @@ __device__ __forceinline__ void stage_load(float2 (&v)[RadixPlan<N>::V], const A
 }
 
 // Stage outputs of stage STAGE to LDS: y[(b/Ns)*Ns*R + b%Ns + q*Ns] = v[u*R + q].
-template <int N, int STAGE, class Acc>
-__device__ __forceinline__ void stage_store(const float2 (&v)[RadixPlan<N>::V], const Acc& acc, int t) {
-    constexpr int R = RadixPlan<N>::r[STAGE];
-    constexpr int NS = span_before<N>(STAGE);
-    constexpr int T = line_threads<N>();
-    constexpr int NB = RadixPlan<N>::V / R;
+template <class PL, int STAGE, class Acc>
+__device__ __forceinline__ void stage_store(const float2 (&v)[PL::V], const Acc& acc, int t) {
+    constexpr int R = PL::r[STAGE];
+    constexpr int NS = span_before<PL>(STAGE);
+    constexpr int T = PL::N / PL::V;
+    constexpr int NB = PL::V / R;
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
         const int b = t + T * u;
@@ -307,39 +372,46 @@ template <bool WAVE> __device__ __forceinline__ void line_sync() {
     }
 }
 
-// Full line transform. On entry v[j] = x[t + j*T]; on exit v[j] = X[t + j*T]
-// (T = N/V) with X[k] = sum_n x[n] exp(SIGN*2*pi*i*n*k/N) (unnormalised). With
-// WAVE = false every thread of the workgroup must call it (it contains
-// workgroup barriers); with WAVE = true the T threads of each line must lie in
-// one wavefront and the line's LDS must be touched by that wavefront only.
-template <int N, int SIGN, bool WAVE = false, class Acc>
-__device__ __forceinline__ void fft_line(float2 (&v)[RadixPlan<N>::V], const Acc& acc, const float2* __restrict__ tw,
-                                         int t) {
-    constexpr int NSTAGE = RadixPlan<N>::ns;
-    constexpr int V = RadixPlan<N>::V;
-    regs_to_stage<V, RadixPlan<N>::r[0]>(v);
-    stage_compute<N, 0, SIGN>(v, tw, t);
-    stage_store<N, 0>(v, acc, t);
+// Full line transform with plan PL. On entry v[j] = x[t + j*T]; on exit
+// v[j] = X[t + j*T] (T = N/V) with X[k] = sum_n x[n] exp(SIGN*2*pi*i*n*k/N)
+// (unnormalised). With WAVE = false every thread of the workgroup must call it
+// (it contains workgroup barriers); with WAVE = true the T threads of each line
+// must lie in one wavefront and the line's LDS must be touched by that
+// wavefront only.
+template <class PL, int SIGN, bool WAVE = false, class Acc>
+__device__ __forceinline__ void fft_line_p(float2 (&v)[PL::V], const Acc& acc, const float2* __restrict__ tw, int t) {
+    constexpr int NSTAGE = PL::ns;
+    constexpr int V = PL::V;
+    regs_to_stage<V, PL::r[0]>(v);
+    stage_compute<PL, 0, SIGN>(v, tw, t);
+    stage_store<PL, 0>(v, acc, t);
     line_sync<WAVE>();
-    stage_load<N, 1>(v, acc, t);
-    stage_compute<N, 1, SIGN>(v, tw, t);
+    stage_load<PL, 1>(v, acc, t);
+    stage_compute<PL, 1, SIGN>(v, tw, t);
     if constexpr (NSTAGE > 2) {
         line_sync<WAVE>();
-        stage_store<N, 1>(v, acc, t);
+        stage_store<PL, 1>(v, acc, t);
         line_sync<WAVE>();
-        stage_load<N, 2>(v, acc, t);
-        stage_compute<N, 2, SIGN>(v, tw, t);
+        stage_load<PL, 2>(v, acc, t);
+        stage_compute<PL, 2, SIGN>(v, tw, t);
     }
     if constexpr (NSTAGE > 3) {
         line_sync<WAVE>();
-        stage_store<N, 2>(v, acc, t);
+        stage_store<PL, 2>(v, acc, t);
         line_sync<WAVE>();
-        stage_load<N, 3>(v, acc, t);
-        stage_compute<N, 3, SIGN>(v, tw, t);
+        stage_load<PL, 3>(v, acc, t);
+        stage_compute<PL, 3, SIGN>(v, tw, t);
     }
-    stage_to_regs<V, RadixPlan<N>::r[NSTAGE - 1]>(v);
+    stage_to_regs<V, PL::r[NSTAGE - 1]>(v);
     // Leave LDS free for the next user of this buffer.
     line_sync<WAVE>();
+}
+
+// The default plan of size N.
+template <int N, int SIGN, bool WAVE = false, class Acc>
+__device__ __forceinline__ void fft_line(float2 (&v)[RadixPlan<N>::V], const Acc& acc, const float2* __restrict__ tw,
+                                         int t) {
+    fft_line_p<RadixPlan<N>, SIGN, WAVE>(v, acc, tw, t);
 }
 
 }  // namespace spr

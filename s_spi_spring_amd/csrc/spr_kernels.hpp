@@ -28,10 +28,15 @@ namespace spr {
 // ---------------------------------------------------------------------------
 constexpr int pow2_floor(int x) { return x >= 2 ? 2 * pow2_floor(x / 2) : 1; }
 
-template <int N> struct Geo {
-    static constexpr int V = RadixPlan<N>::V;             // values per thread
+// Geometry of the row and column passes of size N with radix plan PL_, CCOLS
+// columns per column-pass workgroup (0: the default width), RTHREADS threads per
+// row-pass workgroup and row-pass prefetch mode RPF (-1: SPR_ROW_PREFETCH).
+template <int N, class PL_ = RadixPlan<N>, int CCOLS = 0, int RTHREADS = 256, int RPF = -1> struct GeoT {
+    using PL = PL_;
+    static constexpr int V = PL::V;                       // values per thread
     static constexpr int T = N / V;                       // threads per line
-    static constexpr int ROW_THREADS = 256;
+    static constexpr int ROW_THREADS = RTHREADS;
+    static constexpr int ROW_PREFETCH = RPF;
     static constexpr int RW = ROW_THREADS / T;            // rows per row-pass workgroup
     static constexpr int RS = N + N / 8;                  // padded LDS row length (float2)
 #ifndef SPR_COL_COLUMNS
@@ -47,7 +52,7 @@ template <int N> struct Geo {
 #define SPR_COL_AMP_PREFETCH 1 // load the amplitude with the field, before the forward transform
 #endif
     // columns per column-pass workgroup: a power of two (it divides N)
-    static constexpr int C = pow2_floor(SPR_COL_COLUMNS / N) < N ? pow2_floor(SPR_COL_COLUMNS / N) : N;
+    static constexpr int C = CCOLS ? CCOLS : (pow2_floor(SPR_COL_COLUMNS / N) < N ? pow2_floor(SPR_COL_COLUMNS / N) : N);
     static constexpr bool COL_WAVE = SPR_COL_WAVE && T <= 64;
     static constexpr int COL_THREADS = C * T;
     static constexpr int CS = C + 1;                      // padded LDS row length of a column tile
@@ -60,13 +65,13 @@ template <int N> struct Geo {
     static constexpr int CST = ((N + N / 8) / (2 * RPW) + 1) * (2 * RPW) + RPW;
     // Two column workgroups per CU when their LDS fits (160 KB): waves per SIMD to target.
     static constexpr int COL_TILE = COL_WAVE ? C * CST : N * CS;   // float2 of the column tile
-    static constexpr int COL_LDS = (COL_TILE + TwTable<N>::SIZE) * 8 + 512;
+    static constexpr int COL_LDS = (COL_TILE + TwTableP<PL>::SIZE) * 8 + 512;
 #ifndef SPR_COL_WG_PER_CU
 #define SPR_COL_WG_PER_CU 8
 #endif
     // Row workgroups per CU allowed by LDS (4 waves each, one per SIMD): the
     // register budget the row pass is compiled for.
-    static constexpr int ROW_LDS = (RW * RS + TwTable<N>::SIZE) * 8;
+    static constexpr int ROW_LDS = (RW * RS + TwTableP<PL>::SIZE) * 8;
     static constexpr int ROW_WAVES_PER_EU = (160 * 1024 / ROW_LDS) < 8 ? (160 * 1024 / ROW_LDS) : 8;
     // Workgroups per CU: as many as LDS and the 32 wave slots of a CU allow,
     // at most SPR_COL_WG_PER_CU; the register budget follows from it.
@@ -78,6 +83,67 @@ template <int N> struct Geo {
     static_assert(RW >= 1 && ROW_THREADS % T == 0 && N % RW == 0, "row geometry");
     static_assert(COL_THREADS <= 1024 && N % C == 0, "column geometry");
 };
+template <int N> using Geo = GeoT<N>;
+
+// Variants of the two iteration passes that the autotuner chooses between
+// (spr_autotune): variant 0 is the default geometry. All variants read and
+// write the same HBM layout, so row and column variants combine freely; the
+// column width and the row-pass threads and prefetch run the same operations
+// (up to the compiler's contraction of multiply-adds), a different radix plan
+// changes the rounding of the transforms.
+template <int N, int CV> struct ColVariant { static constexpr bool valid = false; };
+template <int N> struct ColVariant<N, 0> { static constexpr bool valid = true; using G = GeoT<N>; };
+template <int N, int RV> struct RowVariant { static constexpr bool valid = false; };
+template <int N> struct RowVariant<N, 0> { static constexpr bool valid = true; using G = GeoT<N>; };
+#define SPR_COL_VARIANT(N_, I_, ...) \
+    template <> struct ColVariant<N_, I_> { static constexpr bool valid = true; using G = GeoT<N_, __VA_ARGS__>; };
+#define SPR_ROW_VARIANT(N_, I_, ...) \
+    template <> struct RowVariant<N_, I_> { static constexpr bool valid = true; using G = GeoT<N_, __VA_ARGS__>; };
+// column variants: plan, columns per workgroup
+SPR_COL_VARIANT(256, 1, RadixPlan<256>, 16)
+SPR_COL_VARIANT(256, 2, RadixPlan<256>, 8)
+SPR_COL_VARIANT(384, 1, RadixPlan<384>, 8)
+SPR_COL_VARIANT(384, 2, RadixPlan<384>, 32)
+SPR_COL_VARIANT(384, 3, RadixPlan<384>, 4)
+SPR_COL_VARIANT(512, 1, RadixPlan<512>, 8)
+SPR_COL_VARIANT(512, 2, Plan<512, 16, 16, 4, 8>, 16)
+SPR_COL_VARIANT(512, 3, RadixPlan<512>, 4)
+SPR_COL_VARIANT(768, 1, RadixPlan<768>, 16)
+SPR_COL_VARIANT(768, 2, RadixPlan<768>, 4)
+SPR_COL_VARIANT(1024, 1, Plan<1024, 16, 16, 8, 8>, 16)
+SPR_COL_VARIANT(1024, 2, Plan<1024, 16, 16, 8, 8>, 8)
+SPR_COL_VARIANT(1024, 3, RadixPlan<1024>, 4)
+SPR_COL_VARIANT(1024, 4, Plan<1024, 16, 16, 8, 8>, 4)
+// row variants: plan, -, threads per workgroup, prefetch mode
+SPR_ROW_VARIANT(256, 1, RadixPlan<256>, 0, 512)
+SPR_ROW_VARIANT(384, 1, RadixPlan<384>, 0, 512)
+SPR_ROW_VARIANT(512, 1, RadixPlan<512>, 0, 512)
+SPR_ROW_VARIANT(768, 1, RadixPlan<768>, 0, 256, 1)
+SPR_ROW_VARIANT(768, 2, RadixPlan<768>, 0, 512)
+SPR_ROW_VARIANT(1024, 1, Plan<1024, 16, 16, 8, 8>)
+SPR_ROW_VARIANT(1024, 2, Plan<1024, 16, 16, 8, 8>, 0, 256, 1)
+SPR_ROW_VARIANT(1024, 3, Plan<1024, 16, 16, 8, 8>, 0, 512)
+SPR_ROW_VARIANT(2048, 1, Plan<2048, 16, 16, 8, 16>)
+#undef SPR_COL_VARIANT
+#undef SPR_ROW_VARIANT
+
+constexpr int MAX_VARIANTS = 5;
+template <int N> constexpr int col_variants() {
+    int k = 1;
+    if constexpr (ColVariant<N, 1>::valid) k = 2;
+    if constexpr (ColVariant<N, 2>::valid) k = 3;
+    if constexpr (ColVariant<N, 3>::valid) k = 4;
+    if constexpr (ColVariant<N, 4>::valid) k = 5;
+    return k;
+}
+template <int N> constexpr int row_variants() {
+    int k = 1;
+    if constexpr (RowVariant<N, 1>::valid) k = 2;
+    if constexpr (RowVariant<N, 2>::valid) k = 3;
+    if constexpr (RowVariant<N, 3>::valid) k = 4;
+    if constexpr (RowVariant<N, 4>::valid) k = 5;
+    return k;
+}
 
 // Loads and stores of the streamed fields (each value is read and written once
 // per pass). With SPR_NT they carry the non-temporal hint.
@@ -218,17 +284,19 @@ struct RowArgs {
 #define SPR_ROW_PREFETCH 2
 #endif
 
-template <int N, int MODE>
-__global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
-    using G = Geo<N>;
+template <int N, int MODE, int RV = 0>
+__global__ __launch_bounds__((RowVariant<N, RV>::G::ROW_THREADS)) void k_row(RowArgs a) {
+    using G = typename RowVariant<N, RV>::G;
+    using PL = typename G::PL;
     constexpr int V = G::V, T = G::T;
+    constexpr int PREFETCH = G::ROW_PREFETCH >= 0 ? G::ROW_PREFETCH : SPR_ROW_PREFETCH;
     constexpr bool WV = T <= 64;   // every row inside one wavefront: no workgroup barriers
-    __shared__ float2 tw[TwTable<N>::SIZE];
+    __shared__ float2 tw[TwTableP<PL>::SIZE];
     __shared__ float2 lds[G::RW * G::RS];
     const int tid = threadIdx.x;
     const int t = tid % T;
     const int rl = tid / T;
-    tw_fill<N>(tw, a.tw, tid, G::ROW_THREADS);
+    tw_fill_p<PL>(tw, a.tw, tid, G::ROW_THREADS);
     // Block-uniform bases with 32-bit per-thread offsets (SGPR base addressing).
     const size_t blk = (size_t)blockIdx.x * G::RW * N;
     float2* __restrict__ rho = a.rho + blk;
@@ -242,7 +310,7 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
     if constexpr (MODE == ROW_FWD || MODE == ROW_FWD_INPLACE) {
 #pragma unroll
         for (int r = 0; r < V; ++r) v[r] = rho[off + r * T];
-        fft_line<N, +1, WV>(v, acc, tw, t);
+        fft_line_p<PL, +1, WV>(v, acc, tw, t);
         float2* dst = (MODE == ROW_FWD) ? work : rho;
 #pragma unroll
         for (int q = 0; q < V; ++q) dst[off + q * T] = v[q];
@@ -250,7 +318,7 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
         const float2* src = (MODE == ROW_FINAL) ? work : rho;
 #pragma unroll
         for (int r = 0; r < V; ++r) v[r] = src[off + r * T];
-        fft_line<N, -1, WV>(v, acc, tw, t);
+        fft_line_p<PL, -1, WV>(v, acc, tw, t);
 #pragma unroll
         for (int q = 0; q < V; ++q) rho[off + q * T] = make_float2(v[q].x * a.inv_nn, v[q].y * a.inv_nn);
     } else {
@@ -261,26 +329,26 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
 #pragma unroll
         for (int r = 0; r < V; ++r) {
             v[r] = ld_stream(work + off + r * T);
-            if constexpr (SPR_ROW_PREFETCH >= 2) sn[r] = supp[off + r * T];
-            if constexpr (MODE != ROW_ER && SPR_ROW_PREFETCH >= 1) bn[r] = ld_stream(rho + off + r * T);
+            if constexpr (PREFETCH >= 2) sn[r] = supp[off + r * T];
+            if constexpr (MODE != ROW_ER && PREFETCH >= 1) bn[r] = ld_stream(rho + off + r * T);
         }
-        fft_line<N, -1, WV>(v, acc, tw, t);
+        fft_line_p<PL, -1, WV>(v, acc, tw, t);
 #pragma unroll
         for (int q = 0; q < V; ++q) {
             const int i = off + q * T;
             const float2 d = make_float2(v[q].x * a.inv_nn, v[q].y * a.inv_nn);  // rho'
-            const uint8_t s = SPR_ROW_PREFETCH >= 2 ? sn[q] : supp[i];
+            const uint8_t s = PREFETCH >= 2 ? sn[q] : supp[i];
             float2 o;
             if constexpr (MODE == ROW_ER) {
                 const float ts = (s != 0 && !phase_fails(d.x, d.y, a.pc)) ? 1.f : 0.f;
                 o = make_float2(d.x * ts, d.y * ts);
             } else if constexpr (MODE == ROW_HIO) {
-                const float2 b = SPR_ROW_PREFETCH >= 1 ? bn[q] : rho[i];
+                const float2 b = PREFETCH >= 1 ? bn[q] : rho[i];
                 const float ts = (s != 0 && !phase_fails(d.x, d.y, a.pc)) ? 1.f : 0.f;
                 o = make_float2((b.x - a.beta * d.x) * (1.f - ts) + d.x * ts,
                                 (b.y - a.beta * d.y) * (1.f - ts) + d.y * ts);
             } else {  // RAAR
-                const float2 b = SPR_ROW_PREFETCH >= 1 ? bn[q] : rho[i];
+                const float2 b = PREFETCH >= 1 ? bn[q] : rho[i];
                 const float qx = 2.f * d.x - b.x, qy = 2.f * d.y - b.y;
                 o = d;
                 if (s == 0 || phase_fails(qx, qy, a.pc)) {
@@ -291,7 +359,7 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row(RowArgs a) {
             if constexpr (MODE != ROW_ER) st_stream(rho + i, o);
             v[q] = o;
         }
-        fft_line<N, +1, WV>(v, acc, tw, t);
+        fft_line_p<PL, +1, WV>(v, acc, tw, t);
 #pragma unroll
         for (int q = 0; q < V; ++q) st_stream(work + off + q * T, v[q]);
     }
@@ -324,8 +392,7 @@ template <int N> __device__ __forceinline__ size_t amp_offset(const int* amp_of,
 // 128-byte line (C < 16, N >= 1024), the 16/C strips that share those lines
 // are given to workgroups b, b+8, ... of the same XCD, so each line is fetched
 // from HBM into one L2 only.
-template <int N> __device__ __forceinline__ int col_strip(int b) {
-    constexpr int C = Geo<N>::C;
+template <int N, int C = Geo<N>::C> __device__ __forceinline__ int col_strip(int b) {
     constexpr int GS = C < 16 ? 16 / C : 1;     // strips per 128-byte line
     if constexpr (GS == 1 || SPR_COL_XCD == 0) {
         return b;
@@ -344,20 +411,21 @@ template <int N> __device__ __forceinline__ int col_strip(int b) {
 // each wavefront owns whole columns and its transforms synchronise with
 // wavefront fences, reading the amplitude from its transposed copy; at 512^2
 // this form measured 71 us per 64-field launch against 66 us for the default.
-template <int N, int MODE>
-__global__ __launch_bounds__(Geo<N>::COL_THREADS) __attribute__((amdgpu_waves_per_eu(Geo<N>::COL_WAVES_PER_EU)))
-void k_col(ColArgs a) {
-    using G = Geo<N>;
+template <int N, int MODE, int CV = 0>
+__global__ __launch_bounds__((ColVariant<N, CV>::G::COL_THREADS))
+__attribute__((amdgpu_waves_per_eu((ColVariant<N, CV>::G::COL_WAVES_PER_EU)))) void k_col(ColArgs a) {
+    using G = typename ColVariant<N, CV>::G;
+    using PL = typename G::PL;
     constexpr int V = G::V, T = G::T, C = G::C, CS = G::CS;
     constexpr bool WV = G::COL_WAVE;
-    __shared__ float2 tw[TwTable<N>::SIZE];
+    __shared__ float2 tw[TwTableP<PL>::SIZE];
     __shared__ float2 lds[G::COL_TILE];
     const int tid = threadIdx.x;
     const int c = tid % C;
     const int t = tid / C;
-    tw_fill<N>(tw, a.tw, tid, G::COL_THREADS);
+    tw_fill_p<PL>(tw, a.tw, tid, G::COL_THREADS);
     constexpr int TILES = N / C;
-    const int strip = col_strip<N>(blockIdx.x);
+    const int strip = col_strip<N, C>(blockIdx.x);
     const int p = strip / TILES;
     const int x0 = (strip % TILES) * C;
     // Block-uniform bases with 32-bit per-thread offsets (SGPR base addressing).
@@ -377,13 +445,13 @@ void k_col(ColArgs a) {
         const RowAcc acc{lds + cc * CST};
 #pragma unroll
         for (int r = 0; r < V; ++r) v[r] = acc.ldk<T>(tl, r);
-        if constexpr (MODE == COL_PROJ || MODE == COL_FWD) fft_line<N, +1, true>(v, acc, tw, tl);
+        if constexpr (MODE == COL_PROJ || MODE == COL_FWD) fft_line_p<PL, +1, true>(v, acc, tw, tl);
         if constexpr (MODE == COL_PROJ) {
             const float* __restrict__ at = a.amp_t + amp_offset<N>(a.amp_of, p) + (x0 + cc) * N;
 #pragma unroll
             for (int q = 0; q < V; ++q) v[q] = project_modulus(v[q], at[tl + q * T]);
         }
-        if constexpr (MODE == COL_PROJ || MODE == COL_INV) fft_line<N, -1, true>(v, acc, tw, tl);
+        if constexpr (MODE == COL_PROJ || MODE == COL_INV) fft_line_p<PL, -1, true>(v, acc, tw, tl);
 #pragma unroll
         for (int q = 0; q < V; ++q) acc.stk<T>(tl, q, v[q]);
         __syncthreads();
@@ -398,13 +466,13 @@ void k_col(ColArgs a) {
             for (int q = 0; q < V; ++q) am[q] = amp[off + q * T * N];
         }
         __syncthreads();
-        if constexpr (MODE == COL_PROJ || MODE == COL_FWD) fft_line<N, +1>(v, acc, tw, t);
+        if constexpr (MODE == COL_PROJ || MODE == COL_FWD) fft_line_p<PL, +1>(v, acc, tw, t);
         if constexpr (MODE == COL_PROJ) {
 #pragma unroll
             for (int q = 0; q < V; ++q)
                 v[q] = project_modulus(v[q], SPR_COL_AMP_PREFETCH ? am[q] : amp[off + q * T * N]);
         }
-        if constexpr (MODE == COL_PROJ || MODE == COL_INV) fft_line<N, -1>(v, acc, tw, t);
+        if constexpr (MODE == COL_PROJ || MODE == COL_INV) fft_line_p<PL, -1>(v, acc, tw, t);
     }
 #pragma unroll
     for (int q = 0; q < V; ++q) st_stream(f + off + q * T * N, v[q]);

@@ -1,0 +1,20 @@
+// Variant choices of the iteration passes per grid size, written by
+// s_spi_spring_amd/autotune.py from spr_autotune timings on an MI355X (gfx950);
+// sizes not listed use variant 0.
+#pragma once
+
+namespace spr {
+
+struct TunedVariants {
+    int n, col, row;
+};
+constexpr TunedVariants kTuned[] = {
+    {256, 2, 0},
+    {384, 1, 0},
+    {512, 1, 0},
+    {768, 2, 0},
+    {1024, 2, 1},
+    {2048, 0, 1},
+};
+
+}  // namespace spr

@@ -69,6 +69,13 @@ SIGNATURES = {
     "spr_batch_get_population": (_I, [_VP, _I, _VP, _VP]),
     "spr_batch_run": (_I, [_VP, _VP, _I, _I, _F]),
     "spr_batch_evaluate": (_I, [_VP, _F, _VP, _VP, _VP, _VP]),
+    "spr_batch_autotune": (_I, [_VP, _I]),
+    "spr_variants": (_I, [_I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "spr_autotune": (_I, [_VP, _I, _VP, _VP]),
+    "spr_set_variants": (_I, [_VP, _I, _I]),
+    "spr_get_variants": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "spr_default_variants": (_I, [_I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "spr_set_default_variants": (_I, [_I, _I, _I]),
 }
 
 
@@ -135,6 +142,30 @@ def load(path: str = LIB_PATH):
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def variant_counts(n):
+    """(column variants, row variants) compiled for size n."""
+    lib = load()
+    c, r = ctypes.c_int(), ctypes.c_int()
+    if lib.spr_variants(int(n), ctypes.byref(c), ctypes.byref(r)) != 0:
+        raise SpringHipError(lib.spr_last_error(None).decode())
+    return c.value, r.value
+
+
+def default_variants(n):
+    """(column, row) variants that new handles of size n start with."""
+    lib = load()
+    c, r = ctypes.c_int(), ctypes.c_int()
+    if lib.spr_default_variants(int(n), ctypes.byref(c), ctypes.byref(r)) != 0:
+        raise SpringHipError(lib.spr_last_error(None).decode())
+    return c.value, r.value
+
+
+def set_default_variants(n, col, row):
+    lib = load()
+    if lib.spr_set_default_variants(int(n), int(col), int(row)) != 0:
+        raise SpringHipError(lib.spr_last_error(None).decode())
 
 
 class Backend:
@@ -321,6 +352,23 @@ class Backend:
         self._check(self.lib.spr_timing_read(self.h, int(kind), ctypes.byref(ms), ctypes.byref(cnt)))
         return ms.value, cnt.value
 
+    def variants(self):
+        """(column variant, row variant) of this handle."""
+        c, r = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.spr_get_variants(self.h, ctypes.byref(c), ctypes.byref(r)))
+        return c.value, r.value
+
+    def set_variants(self, col, row):
+        self._check(self.lib.spr_set_variants(self.h, int(col), int(row)))
+
+    def autotune(self, reps=5):
+        """Time every variant on the resident population and keep the fastest;
+        returns (column ms per variant, row ms per variant)."""
+        nc, nr = variant_counts(self.n)
+        cm, rm = np.zeros(nc, np.float32), np.zeros(nr, np.float32)
+        self._check(self.lib.spr_autotune(self.h, int(reps), _ptr(cm), _ptr(rm)))
+        return cm, rm
+
     def stream(self):
         s = ctypes.c_void_p()
         self._check(self.lib.spr_stream(self.h, ctypes.byref(s)))
@@ -391,6 +439,10 @@ class Batch:
         ops = make_ops(sequence)
         self._check(self.lib.spr_batch_run(self.h, ctypes.cast(ops, ctypes.c_void_p), len(sequence), int(nrep),
                                            float(reality)))
+
+    def autotune(self, reps=5):
+        """spr_autotune on every size class (see Backend.autotune)."""
+        self._check(self.lib.spr_batch_autotune(self.h, int(reps)))
 
     def evaluate(self, reality=0.0):
         """Dict of per-pattern arrays: error_fourier, error_real, complexsum, modulesum."""
