@@ -232,6 +232,7 @@ def batch_bench(args, log):
         ", ".join(f"{n}^2 x {sizes.count(n)}" for n in sorted(set(sizes))))
     b = hipcore.Batch(amps, [P] * len(amps), device=0)
     b.graphs(not args.no_graphs)
+    b.concurrent(args.batch_concurrent)
     for k, a in enumerate(amps):
         n = a.shape[0]
         if n not in starts:   # one random start per size, scaled per pattern
@@ -268,7 +269,7 @@ def batch_bench(args, log):
                                   f"{'one HIP graph launch' if not args.no_graphs else 'direct launches'} per block",
                       "patterns": len(amps), "sizes": {str(n): sizes.count(n) for n in sorted(set(sizes))},
                       "population": P, "individuals": b.total, "iterations_per_step": args.batch_block,
-                      "graphs": not args.no_graphs},
+                      "graphs": not args.no_graphs, "size_classes": "concurrent" if args.batch_concurrent else "in turn"},
            "iteration_frac": round(frac, 4),
            "fourier_error_median_after": float(np.median(ef))}
     print(json.dumps(out), flush=True)
@@ -303,6 +304,8 @@ def main():
     ap.add_argument("--batch-pop", type=int, default=32)
     ap.add_argument("--batch-block", type=int, default=50)
     ap.add_argument("--no-graphs", action="store_true", help="batch: direct launches instead of HIP graphs")
+    ap.add_argument("--batch-concurrent", action="store_true",
+                    help="batch: run the size classes of a block side by side instead of in turn")
     ap.add_argument("--summarize-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write the HBM traffic summary from two rocprofv3 --pmc runs of "
                          "`bench.py --calibrate MIB` (FETCH_SIZE, WRITE_SIZE) and exit; no GPU needed")

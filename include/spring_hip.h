@@ -302,7 +302,8 @@ int spr_set_default_variants(int n, int col, int row);
  * covers the whole size class, reading each individual's own amplitude.
  * spr_batch_run applies the op list of spr_run to every individual of every
  * pattern; an ER/HIO/RAAR block runs as one HIP graph launch whose branches
- * (one per size) run concurrently, captured on first use of that block's
+ * (one per size) run one after another, or side by side after
+ * spr_batch_concurrent(b, 1), captured on first use of that block's
  * (algorithm, iterations, beta, reality) and replayed afterwards;
  * spr_batch_graphs(b, 0) issues the same launches directly instead.
  * Populations start at rho = 0 and empty supports; spr_batch_set_population
@@ -316,6 +317,7 @@ int spr_batch_create(spr_batch** out, int npatterns, const int* n, const int* po
 int spr_batch_destroy(spr_batch* b);
 const char* spr_batch_last_error(const spr_batch* b);
 int spr_batch_graphs(spr_batch* b, int enable);
+int spr_batch_concurrent(spr_batch* b, int enable);
 int spr_batch_set_population(spr_batch* b, int pattern, const float* rho, const int32_t* support);
 int spr_batch_get_population(spr_batch* b, int pattern, float* rho, int32_t* support);
 int spr_batch_run(spr_batch* b, const spr_op* ops, int nops, int nrep, float reality);

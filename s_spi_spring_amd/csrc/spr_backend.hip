@@ -38,6 +38,7 @@ struct spr_ctx {
     float* amp_t = nullptr;       // amp transposed (read along columns by the column pass)
     int npat = 1;                 // patterns sharing this handle (one amplitude each)
     int colvar = 0, rowvar = 0;   // variants of the column and row iteration passes
+    float2* twtab[MAX_VARIANTS] = {};  // global twiddle tables of the column variants that read one
     int* amp_of = nullptr;        // slot -> pattern when npat > 1, else NULL
     float2* tw = nullptr;
     // reduction scratch
@@ -183,20 +184,47 @@ template <int N> int launch_row(spr_ctx* c, int mode, const RowArgs& a, int cnt,
     return fail(c, SPR_ERR_ARG, "bad row variant");
 }
 
+// Column passes: the projection pass in every variant, the plain transforms
+// (2-D FT / IFT, off the iteration path) in variant 0.
 template <int N, int CV> int launch_col_v(spr_ctx* c, int mode, float2* f, int cnt, hipStream_t s) {
     using G = typename ColVariant<N, CV>::G;
-    const ColArgs a{f, c->amp, c->tw, c->amp_t, c->amp_of};
+    if (G::TW_GLOBAL && !c->twtab[CV]) return fail(c, SPR_ERR_STATE, "twiddle table missing");
+    const ColArgs a{f, c->amp, c->tw, c->amp_t, c->amp_of, c->twtab[CV]};
     const dim3 grid((unsigned)((size_t)cnt * (N / G::C))), block(G::COL_THREADS);
     switch (mode) {
         case COL_PROJ: hipLaunchKernelGGL((k_col<N, COL_PROJ, CV>), grid, block, 0, s, a); break;
-        case COL_FWD: hipLaunchKernelGGL((k_col<N, COL_FWD, CV>), grid, block, 0, s, a); break;
-        case COL_INV: hipLaunchKernelGGL((k_col<N, COL_INV, CV>), grid, block, 0, s, a); break;
+        case COL_FWD:
+            if constexpr (CV == 0) hipLaunchKernelGGL((k_col<N, COL_FWD, 0>), grid, block, 0, s, a);
+            break;
+        case COL_INV:
+            if constexpr (CV == 0) hipLaunchKernelGGL((k_col<N, COL_INV, 0>), grid, block, 0, s, a);
+            break;
         default: return fail(c, SPR_ERR_ARG, "bad column mode");
     }
     SPR_HIP(c, hipGetLastError());
     return SPR_OK;
 }
+
+// Global twiddle tables of every column variant of size N that reads one.
+template <int N, int CV = 0> int build_twtabs(spr_ctx* c) {
+    if constexpr (CV < MAX_VARIANTS) {
+        if constexpr (ColVariant<N, CV>::valid) {
+            using G = typename ColVariant<N, CV>::G;
+            if constexpr (G::TW_GLOBAL) {
+                constexpr int SIZE = TwTableP<typename G::PL>::SIZE;
+                SPR_HIP(c, hipMalloc(&c->twtab[CV], SIZE * sizeof(float2)));
+                hipLaunchKernelGGL((k_build_twtab<typename G::PL>), dim3(1), dim3(256), 0, c->stream, c->twtab[CV],
+                                   (const float2*)c->tw);
+                SPR_HIP(c, hipGetLastError());
+            }
+        }
+        return build_twtabs<N, CV + 1>(c);
+    }
+    return SPR_OK;
+}
+
 template <int N> int launch_col(spr_ctx* c, int mode, float2* f, int cnt, hipStream_t s) {
+    if (mode != COL_PROJ) return launch_col_v<N, 0>(c, mode, f, cnt, s);
     switch (c->colvar) {
         case 0: return launch_col_v<N, 0>(c, mode, f, cnt, s);
         case 1:
@@ -210,6 +238,15 @@ template <int N> int launch_col(spr_ctx* c, int mode, float2* f, int cnt, hipStr
             break;
         case 4:
             if constexpr (ColVariant<N, 4>::valid) return launch_col_v<N, 4>(c, mode, f, cnt, s);
+            break;
+        case 5:
+            if constexpr (ColVariant<N, 5>::valid) return launch_col_v<N, 5>(c, mode, f, cnt, s);
+            break;
+        case 6:
+            if constexpr (ColVariant<N, 6>::valid) return launch_col_v<N, 6>(c, mode, f, cnt, s);
+            break;
+        case 7:
+            if constexpr (ColVariant<N, 7>::valid) return launch_col_v<N, 7>(c, mode, f, cnt, s);
             break;
     }
     return fail(c, SPR_ERR_ARG, "bad column variant");
@@ -960,6 +997,20 @@ static int create_context(spr_ctx** out, int n, int capacity, int npat, const fl
     }
     if ((e = hipMemcpy(c->tw, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice)) != hipSuccess)
         return cleanup_fail("upload twiddles", e);
+    {
+        int rc = SPR_ERR_ARG;
+        switch (n) {
+#define SPR_TWTAB_CASE(N_) \
+    case N_: rc = build_twtabs<N_>(c); break;
+            SPR_SIZES(SPR_TWTAB_CASE)
+#undef SPR_TWTAB_CASE
+        }
+        if (rc != SPR_OK || (e = hipStreamSynchronize(c->stream)) != hipSuccess) {
+            g_create_error = "twiddle tables: " + c->err;
+            spr_destroy(c);
+            return SPR_ERR_HIP;
+        }
+    }
     for (int k = 0; k < npat; ++k) {
         if ((e = hipMemcpy(c->amp + k * np, amps[k], np * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess)
             return cleanup_fail("upload amplitude", e);
@@ -988,6 +1039,7 @@ int spr_destroy(spr_ctx* c) {
     (void)hipFree(c->amp);
     (void)hipFree(c->amp_t);
     (void)hipFree(c->amp_of);
+    for (float2* t : c->twtab) (void)hipFree(t);
     (void)hipFree(c->tw);
     (void)hipFree(c->work2);
     (void)hipFree(c->partial);
@@ -1782,6 +1834,7 @@ struct spr_batch {
     hipStream_t origin = nullptr;                     // stream the block graphs are launched on
     hipEvent_t fork = nullptr;
     bool graphs = true;
+    bool concurrent = false;                          // size classes of a block side by side, else in turn
     std::map<std::tuple<int, int, unsigned, unsigned>, hipGraphExec_t> blocks;
     std::string err;
 };
@@ -1845,7 +1898,9 @@ static int batch_block_graph(spr_batch* b, int alg, int niter, float beta, float
     if (hipEventRecord(b->fork, b->origin) != hipSuccess) rc = bfail(b, SPR_ERR_HIP, "capture fork failed");
     for (size_t i = 0; rc == SPR_OK && i < b->classes.size(); ++i) {
         auto& k = b->classes[i];
-        if (hipStreamWaitEvent(k.ctx->stream, b->fork, 0) != hipSuccess) {
+        // concurrent: every class branches from the fork; in turn: from the previous class
+        hipEvent_t from = (b->concurrent || i == 0) ? b->fork : b->classes[i - 1].done;
+        if (hipStreamWaitEvent(k.ctx->stream, from, 0) != hipSuccess) {
             rc = bfail(b, SPR_ERR_HIP, "capture fork failed");
             break;
         }
@@ -1871,9 +1926,22 @@ static int batch_block_graph(spr_batch* b, int alg, int niter, float beta, float
     return SPR_OK;
 }
 
+// Class i's stream waits for class i-1's work queued so far (in-turn order).
+static int batch_chain(spr_batch* b, size_t i) {
+    if (i == 0 || b->concurrent) return SPR_OK;
+    auto& prev = b->classes[i - 1];
+    SPR_BHIP(b, hipEventRecord(prev.done, prev.ctx->stream));
+    SPR_BHIP(b, hipStreamWaitEvent(b->classes[i].ctx->stream, prev.done, 0));
+    return SPR_OK;
+}
+
 static int batch_iterate(spr_batch* b, int alg, int niter, float beta, float reality) {
     if (!b->graphs) {
-        for (auto& k : b->classes) SPR_TRY(bclass(b, k, queue_iterate(k.ctx, alg, niter, beta, reality)));
+        for (size_t i = 0; i < b->classes.size(); ++i) {
+            SPR_TRY(batch_chain(b, i));
+            auto& k = b->classes[i];
+            SPR_TRY(bclass(b, k, queue_iterate(k.ctx, alg, niter, beta, reality)));
+        }
         return SPR_OK;
     }
     hipGraphExec_t x = nullptr;
@@ -1966,6 +2034,17 @@ const char* spr_batch_last_error(const spr_batch* b) { return b ? b->err.c_str()
 int spr_batch_graphs(spr_batch* b, int enable) {
     if (!b) return bfail(b, SPR_ERR_ARG, "NULL handle");
     b->graphs = enable != 0;
+    return SPR_OK;
+}
+
+int spr_batch_concurrent(spr_batch* b, int enable) {
+    if (!b) return bfail(b, SPR_ERR_ARG, "NULL handle");
+    SPR_TRY(batch_sync(b));
+    if (b->concurrent != (enable != 0)) {
+        for (auto& kv : b->blocks) (void)hipGraphExecDestroy(kv.second);
+        b->blocks.clear();
+    }
+    b->concurrent = enable != 0;
     return SPR_OK;
 }
 

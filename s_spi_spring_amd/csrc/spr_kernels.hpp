@@ -30,9 +30,14 @@ constexpr int pow2_floor(int x) { return x >= 2 ? 2 * pow2_floor(x / 2) : 1; }
 
 // Geometry of the row and column passes of size N with radix plan PL_, CCOLS
 // columns per column-pass workgroup (0: the default width), RTHREADS threads per
-// row-pass workgroup and row-pass prefetch mode RPF (-1: SPR_ROW_PREFETCH).
-template <int N, class PL_ = RadixPlan<N>, int CCOLS = 0, int RTHREADS = 256, int RPF = -1> struct GeoT {
+// row-pass workgroup, row-pass prefetch mode RPF (-1: SPR_ROW_PREFETCH), PAD
+// float2 of padding per row of the column tile, and TWG = 1 to read the column
+// pass's twiddle table from global memory (through the caches) instead of
+// building it in LDS.
+template <int N, class PL_ = RadixPlan<N>, int CCOLS = 0, int RTHREADS = 256, int RPF = -1, int PAD = 1, int TWG = 0>
+struct GeoT {
     using PL = PL_;
+    static constexpr bool TW_GLOBAL = TWG != 0;
     static constexpr int V = PL::V;                       // values per thread
     static constexpr int T = N / V;                       // threads per line
     static constexpr int ROW_THREADS = RTHREADS;
@@ -55,7 +60,7 @@ template <int N, class PL_ = RadixPlan<N>, int CCOLS = 0, int RTHREADS = 256, in
     static constexpr int C = CCOLS ? CCOLS : (pow2_floor(SPR_COL_COLUMNS / N) < N ? pow2_floor(SPR_COL_COLUMNS / N) : N);
     static constexpr bool COL_WAVE = SPR_COL_WAVE && T <= 64;
     static constexpr int COL_THREADS = C * T;
-    static constexpr int CS = C + 1;                      // padded LDS row length of a column tile
+    static constexpr int CS = C + PAD;                    // padded LDS row length of a column tile
     // Column-major LDS tile of the wavefront-owned column pass (T <= 64): column
     // c, row i at c*CST + i + i/8. Within a column this is the row pass's padded
     // layout; the column stride CST = RPW (mod 2*RPW) float2, RPW = 64/C rows
@@ -65,7 +70,7 @@ template <int N, class PL_ = RadixPlan<N>, int CCOLS = 0, int RTHREADS = 256, in
     static constexpr int CST = ((N + N / 8) / (2 * RPW) + 1) * (2 * RPW) + RPW;
     // Two column workgroups per CU when their LDS fits (160 KB): waves per SIMD to target.
     static constexpr int COL_TILE = COL_WAVE ? C * CST : N * CS;   // float2 of the column tile
-    static constexpr int COL_LDS = (COL_TILE + TwTableP<PL>::SIZE) * 8 + 512;
+    static constexpr int COL_LDS = (COL_TILE + (TWG ? 0 : TwTableP<PL>::SIZE)) * 8 + 512;
 #ifndef SPR_COL_WG_PER_CU
 #define SPR_COL_WG_PER_CU 8
 #endif
@@ -114,6 +119,18 @@ SPR_COL_VARIANT(1024, 1, Plan<1024, 16, 16, 8, 8>, 16)
 SPR_COL_VARIANT(1024, 2, Plan<1024, 16, 16, 8, 8>, 8)
 SPR_COL_VARIANT(1024, 3, RadixPlan<1024>, 4)
 SPR_COL_VARIANT(1024, 4, Plan<1024, 16, 16, 8, 8>, 4)
+// (plan, columns, -, -, padding, global twiddles): tiles under 80 KB, two workgroups per CU
+SPR_COL_VARIANT(384, 4, RadixPlan<384>, 16, 256, -1, 1, 1)
+SPR_COL_VARIANT(512, 4, RadixPlan<512>, 16, 256, -1, 1, 1)
+SPR_COL_VARIANT(512, 5, RadixPlan<512>, 8, 256, -1, 1, 1)
+SPR_COL_VARIANT(768, 3, RadixPlan<768>, 8, 256, -1, 1, 1)
+SPR_COL_VARIANT(768, 4, RadixPlan<768>, 8, 256, -1, 0, 1)
+SPR_COL_VARIANT(768, 5, RadixPlan<768>, 4, 256, -1, 1, 1)
+SPR_COL_VARIANT(1024, 5, Plan<1024, 16, 16, 8, 8>, 8, 256, -1, 1, 1)
+SPR_COL_VARIANT(1024, 6, Plan<1024, 16, 16, 8, 8>, 8, 256, -1, 0, 0)
+SPR_COL_VARIANT(1024, 7, RadixPlan<1024>, 8, 256, -1, 1, 1)
+SPR_COL_VARIANT(2048, 1, RadixPlan<2048>, 4, 256, -1, 0, 1)
+SPR_COL_VARIANT(2048, 2, Plan<2048, 16, 16, 8, 16>, 8, 256, -1, 0, 1)
 // row variants: plan, -, threads per workgroup, prefetch mode
 SPR_ROW_VARIANT(256, 1, RadixPlan<256>, 0, 512)
 SPR_ROW_VARIANT(384, 1, RadixPlan<384>, 0, 512)
@@ -127,13 +144,16 @@ SPR_ROW_VARIANT(2048, 1, Plan<2048, 16, 16, 8, 16>)
 #undef SPR_COL_VARIANT
 #undef SPR_ROW_VARIANT
 
-constexpr int MAX_VARIANTS = 5;
+constexpr int MAX_VARIANTS = 8;
 template <int N> constexpr int col_variants() {
     int k = 1;
     if constexpr (ColVariant<N, 1>::valid) k = 2;
     if constexpr (ColVariant<N, 2>::valid) k = 3;
     if constexpr (ColVariant<N, 3>::valid) k = 4;
     if constexpr (ColVariant<N, 4>::valid) k = 5;
+    if constexpr (ColVariant<N, 5>::valid) k = 6;
+    if constexpr (ColVariant<N, 6>::valid) k = 7;
+    if constexpr (ColVariant<N, 7>::valid) k = 8;
     return k;
 }
 template <int N> constexpr int row_variants() {
@@ -380,6 +400,7 @@ struct ColArgs {
     const float2* tw;
     const float* amp_t; // the amplitudes transposed, amp_t[x*N + y] = amp[y*N + x]
     const int* amp_of;  // field -> pattern (NULL: every field belongs to pattern 0)
+    const float2* twtab;  // the plan's twiddle table in global memory (variants with TW_GLOBAL)
 };
 
 // Offset of the amplitude of field p (block-uniform: one scalar load).
@@ -418,12 +439,14 @@ __attribute__((amdgpu_waves_per_eu((ColVariant<N, CV>::G::COL_WAVES_PER_EU)))) v
     using PL = typename G::PL;
     constexpr int V = G::V, T = G::T, C = G::C, CS = G::CS;
     constexpr bool WV = G::COL_WAVE;
-    __shared__ float2 tw[TwTableP<PL>::SIZE];
+    __shared__ float2 twl[G::TW_GLOBAL ? 1 : TwTableP<PL>::SIZE];
     __shared__ float2 lds[G::COL_TILE];
     const int tid = threadIdx.x;
     const int c = tid % C;
     const int t = tid / C;
-    tw_fill_p<PL>(tw, a.tw, tid, G::COL_THREADS);
+    const float2* tw = twl;
+    if constexpr (G::TW_GLOBAL) tw = a.twtab;
+    else tw_fill_p<PL>(twl, a.tw, tid, G::COL_THREADS);
     constexpr int TILES = N / C;
     const int strip = col_strip<N, C>(blockIdx.x);
     const int p = strip / TILES;
@@ -476,6 +499,16 @@ __attribute__((amdgpu_waves_per_eu((ColVariant<N, CV>::G::COL_WAVES_PER_EU)))) v
     }
 #pragma unroll
     for (int q = 0; q < V; ++q) st_stream(f + off + q * T * N, v[q]);
+}
+
+}  // namespace spr
+
+namespace spr {
+
+// The twiddle table of plan PL in global memory (one workgroup), for the
+// column-pass variants that read it from there.
+template <class PL> __global__ void k_build_twtab(float2* out, const float2* __restrict__ root) {
+    tw_fill_p<PL>(out, root, threadIdx.x, blockDim.x);
 }
 
 }  // namespace spr

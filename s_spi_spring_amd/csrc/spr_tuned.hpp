@@ -13,8 +13,8 @@ constexpr TunedVariants kTuned[] = {
     {384, 1, 0},
     {512, 1, 0},
     {768, 2, 0},
-    {1024, 2, 1},
-    {2048, 0, 1},
+    {1024, 6, 1},
+    {2048, 1, 1},
 };
 
 }  // namespace spr

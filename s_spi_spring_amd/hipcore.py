@@ -65,6 +65,7 @@ SIGNATURES = {
     "spr_batch_destroy": (_I, [_VP]),
     "spr_batch_last_error": (ctypes.c_char_p, [_VP]),
     "spr_batch_graphs": (_I, [_VP, _I]),
+    "spr_batch_concurrent": (_I, [_VP, _I]),
     "spr_batch_set_population": (_I, [_VP, _I, _VP, _VP]),
     "spr_batch_get_population": (_I, [_VP, _I, _VP, _VP]),
     "spr_batch_run": (_I, [_VP, _VP, _I, _I, _F]),
@@ -421,6 +422,10 @@ class Batch:
 
     def graphs(self, enable: bool):
         self._check(self.lib.spr_batch_graphs(self.h, 1 if enable else 0))
+
+    def concurrent(self, enable: bool):
+        """Size classes of an iteration block side by side (True) or in turn (False, the default)."""
+        self._check(self.lib.spr_batch_concurrent(self.h, 1 if enable else 0))
 
     def set_population(self, k, rho, support=None):
         n, P = int(self.sizes[k]), int(self.populations[k])

@@ -48,10 +48,13 @@ def test_batch_equals_single_pattern_handles(hip):
     b.close()
 
 
-def test_graph_replays_equal_direct_launches(hip):
+@pytest.mark.parametrize("concurrent", [False, True])
+def test_graph_replays_equal_direct_launches(hip, concurrent):
     cs = cases()
     a = hipcore.Batch([c[0] for c in cs], POPS)
     d = hipcore.Batch([c[0] for c in cs], POPS)
+    a.concurrent(concurrent)
+    d.concurrent(not concurrent)
     d.graphs(False)
     for k, (amp, rho, supp) in enumerate(cs):
         a.set_population(k, rho, supp)
