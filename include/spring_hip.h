@@ -276,7 +276,8 @@ int spr_stream(spr_ctx* ctx, void** stream);
  * spr_variants: number of column and row variants of size n.
  * spr_autotune: times `reps` launches of every variant on the handle's
  *   resident population (densities and supports are left unchanged), keeps
- *   the fastest pair for this handle and makes it the default of handles of
+ *   the fastest pair (per pass, the lowest-numbered variant within 1.5 % of
+ *   the fastest) for this handle and makes it the default of handles of
  *   that size created later in the process; col_ms / row_ms (NULL or one
  *   entry per variant) receive the mean launch durations in ms.
  * spr_set_variants / spr_get_variants: one handle's pair.

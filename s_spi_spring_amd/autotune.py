@@ -4,7 +4,7 @@ For every supported size from 256 up, a population of synthetic patterns of
 about 2^24 pixels in total (64 individuals at 512^2) is made resident and
 spr_autotune times every column and row variant on it (`--reps` launches each,
 `--rounds` rounds; a variant's time is its median over the rounds). The fastest
-pair per size is written as the compiled default table (spr_tuned.hpp) and the
+variant of each pass (the lowest-numbered one within 1.5 % of the fastest) per size is written as the compiled default table (spr_tuned.hpp) and the
 timings as JSON; rebuild the library afterwards.
 
     python -m s_spi_spring_amd.autotune --header OUT.hpp --json OUT.json
@@ -19,6 +19,16 @@ import sys
 import numpy as np
 
 SIZES = (256, 384, 512, 768, 1024, 2048)
+
+
+TIE = 1.015
+
+
+def pick(ms):
+    """Lowest-numbered variant within TIE of the fastest (near-ties are timing
+    noise; the lower index keeps the table stable between tuning runs)."""
+    ms = np.asarray(ms)
+    return int(np.nonzero(ms <= ms.min() * TIE)[0][0])
 
 
 def tune_size(n, reps, rounds, log):
@@ -37,7 +47,7 @@ def tune_size(n, reps, rounds, log):
         rms.append(rm)
     be.close()
     cm, rm = np.median(cms, 0), np.median(rms, 0)
-    best = (int(np.argmin(cm)), int(np.argmin(rm)))
+    best = (pick(cm), pick(rm))
     log(f"n={n} P={P}: column ms {np.round(cm, 4).tolist()} row ms {np.round(rm, 4).tolist()} -> {best}")
     return {"n": n, "population": P, "column_ms": cm.tolist(), "row_ms": rm.tolist(),
             "column_variant": best[0], "row_variant": best[1]}

@@ -856,29 +856,33 @@ int autotune(spr_ctx* c, int reps, float* col_ms, float* row_ms) {
         (void)hipEventDestroy(e0);
         return fail(c, SPR_ERR_HIP, "hipEventCreate failed");
     }
-    VariantPair best;
-    float best_c = 0.f, best_r = 0.f;
+    float tc[MAX_VARIANTS] = {}, tr[MAX_VARIANTS] = {};
     int rc = SPR_OK;
     for (int v = 0; v < nc && rc == SPR_OK; ++v) {
-        float ms = 0.f;
         c->colvar = v;
-        rc = time_pass(c, true, ra, reps, e0, e1, &ms);
-        if (col_ms) col_ms[v] = ms;
-        if (rc == SPR_OK && (v == 0 || ms < best_c)) {
-            best_c = ms;
-            best.col = v;
-        }
+        rc = time_pass(c, true, ra, reps, e0, e1, &tc[v]);
+        if (col_ms) col_ms[v] = tc[v];
     }
     c->colvar = keep.col;
     for (int v = 0; v < nr && rc == SPR_OK; ++v) {
-        float ms = 0.f;
         c->rowvar = v;
-        rc = time_pass(c, false, ra, reps, e0, e1, &ms);
-        if (row_ms) row_ms[v] = ms;
-        if (rc == SPR_OK && (v == 0 || ms < best_r)) {
-            best_r = ms;
-            best.row = v;
-        }
+        rc = time_pass(c, false, ra, reps, e0, e1, &tr[v]);
+        if (row_ms) row_ms[v] = tr[v];
+    }
+    // The lowest-numbered variant within TIE of the fastest: near-ties are timing
+    // noise, and preferring the lower index keeps the choice stable between runs.
+    constexpr float TIE = 1.015f;
+    auto pick = [](const float* t, int cnt) {
+        float m = t[0];
+        for (int v = 1; v < cnt; ++v) m = std::min(m, t[v]);
+        for (int v = 0; v < cnt; ++v)
+            if (t[v] <= m * TIE) return v;
+        return 0;
+    };
+    VariantPair best;
+    if (rc == SPR_OK) {
+        best.col = pick(tc, nc);
+        best.row = pick(tr, nr);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

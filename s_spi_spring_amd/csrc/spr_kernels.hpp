@@ -34,16 +34,18 @@ constexpr int pow2_floor(int x) { return x >= 2 ? 2 * pow2_floor(x / 2) : 1; }
 // float2 of padding per row of the column tile, and TWG = 1 to read the column
 // pass's twiddle table from global memory (through the caches) instead of
 // building it in LDS.
-template <int N, class PL_ = RadixPlan<N>, int CCOLS = 0, int RTHREADS = 256, int RPF = -1, int PAD = 1, int TWG = 0>
+template <int N, class PL_ = RadixPlan<N>, int CCOLS = 0, int RTHREADS = 256, int RPF = -1, int PAD = 1, int TWG = 0,
+          int RSH = 3>
 struct GeoT {
     using PL = PL_;
     static constexpr bool TW_GLOBAL = TWG != 0;
+    static constexpr int ROW_PAD_SHIFT = RSH;            // row LDS: one float2 of padding per 2^RSH
     static constexpr int V = PL::V;                       // values per thread
     static constexpr int T = N / V;                       // threads per line
     static constexpr int ROW_THREADS = RTHREADS;
     static constexpr int ROW_PREFETCH = RPF;
     static constexpr int RW = ROW_THREADS / T;            // rows per row-pass workgroup
-    static constexpr int RS = N + N / 8;                  // padded LDS row length (float2)
+    static constexpr int RS = N + (N >> RSH);             // padded LDS row length (float2)
 #ifndef SPR_COL_COLUMNS
 #define SPR_COL_COLUMNS 8192   // columns x rows of a column-pass tile
 #endif
@@ -141,6 +143,9 @@ SPR_ROW_VARIANT(1024, 1, Plan<1024, 16, 16, 8, 8>)
 SPR_ROW_VARIANT(1024, 2, Plan<1024, 16, 16, 8, 8>, 0, 256, 1)
 SPR_ROW_VARIANT(1024, 3, Plan<1024, 16, 16, 8, 8>, 0, 512)
 SPR_ROW_VARIANT(2048, 1, Plan<2048, 16, 16, 8, 16>)
+// (plan, -, threads, prefetch, -, -, padding period 2^4)
+SPR_ROW_VARIANT(1024, 4, Plan<1024, 16, 16, 8, 8>, 0, 256, -1, 1, 0, 4)
+SPR_ROW_VARIANT(2048, 2, Plan<2048, 16, 16, 8, 16>, 0, 256, -1, 1, 0, 4)
 #undef SPR_COL_VARIANT
 #undef SPR_ROW_VARIANT
 
@@ -191,21 +196,25 @@ __device__ __forceinline__ void st_stream(float2* p, float2 v) {
     }
 }
 
-struct RowAcc {
+// LDS row of a line transform with one float2 of padding after every 2^SH
+// elements (SH = 3 by default; 4 suits plans whose first radix is 16, whose
+// first-stage stores stride by 16 elements).
+template <int SH> struct RowAccT {
     float2* base;
-    __device__ __forceinline__ float2 ld(int i) const { return base[i + (i >> 3)]; }
-    __device__ __forceinline__ void st(int i, float2 v) const { base[i + (i >> 3)] = v; }
-    // Element i0 + k*STEP for STEP a multiple of 8: one base register plus
-    // constant offsets, since (i0 + k*STEP) >> 3 = (i0 >> 3) + k*STEP/8.
+    __device__ __forceinline__ float2 ld(int i) const { return base[i + (i >> SH)]; }
+    __device__ __forceinline__ void st(int i, float2 v) const { base[i + (i >> SH)] = v; }
+    // Element i0 + k*STEP for STEP a multiple of 2^SH: one base register plus
+    // constant offsets, since (i0 + k*STEP) >> SH = (i0 >> SH) + k*STEP/2^SH.
     template <int STEP> __device__ __forceinline__ float2 ldk(int i0, int k) const {
-        static_assert(STEP % 8 == 0, "stride");
-        return base[i0 + (i0 >> 3) + k * (STEP + STEP / 8)];
+        static_assert(STEP % (1 << SH) == 0, "stride");
+        return base[i0 + (i0 >> SH) + k * (STEP + (STEP >> SH))];
     }
     template <int STEP> __device__ __forceinline__ void stk(int i0, int k, float2 v) const {
-        static_assert(STEP % 8 == 0, "stride");
-        base[i0 + (i0 >> 3) + k * (STEP + STEP / 8)] = v;
+        static_assert(STEP % (1 << SH) == 0, "stride");
+        base[i0 + (i0 >> SH) + k * (STEP + (STEP >> SH))] = v;
     }
 };
+using RowAcc = RowAccT<3>;
 
 template <int CS> struct ColAcc {
     float2* base;
@@ -323,7 +332,7 @@ __global__ __launch_bounds__((RowVariant<N, RV>::G::ROW_THREADS)) void k_row(Row
     float2* __restrict__ work = a.work + blk;
     const uint8_t* __restrict__ supp = a.supp + blk;
     const int off = rl * N + t;
-    const RowAcc acc{lds + rl * G::RS};
+    const RowAccT<G::ROW_PAD_SHIFT> acc{lds + rl * G::RS};
     float2 v[V];
     __syncthreads();
 

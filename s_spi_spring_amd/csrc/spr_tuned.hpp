@@ -14,7 +14,7 @@ constexpr TunedVariants kTuned[] = {
     {512, 1, 0},
     {768, 2, 0},
     {1024, 6, 1},
-    {2048, 1, 1},
+    {2048, 1, 2},
 };
 
 }  // namespace spr

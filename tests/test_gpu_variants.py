@@ -24,7 +24,8 @@ SAME_PLAN = [(256, "col", 1, 0), (256, "col", 2, 0), (384, "col", 1, 0), (384, "
              (1024, "col", 3, 0), (1024, "col", 4, 1), (256, "row", 1, 0), (384, "row", 1, 0), (512, "row", 1, 0),
              (768, "row", 1, 0), (768, "row", 2, 0), (1024, "row", 2, 1), (1024, "row", 3, 1),
              (384, "col", 4, 0), (512, "col", 4, 0), (512, "col", 5, 0), (768, "col", 3, 0), (768, "col", 4, 0),
-             (768, "col", 5, 0), (1024, "col", 5, 1), (1024, "col", 6, 1), (1024, "col", 7, 0), (2048, "col", 1, 0)]
+             (768, "col", 5, 0), (1024, "col", 5, 1), (1024, "col", 6, 1), (1024, "col", 7, 0), (2048, "col", 1, 0),
+             (1024, "row", 4, 1), (2048, "row", 2, 1)]
 
 
 def pairs(n):
@@ -78,7 +79,8 @@ def test_autotune_keeps_state_and_picks_the_fastest(hip):
     assert np.all(cm > 0) and np.all(rm > 0)
     got, s = be.get_population()
     assert np.array_equal(got, rho) and np.array_equal(s, supp)
-    assert be.variants() == (int(np.argmin(cm)), int(np.argmin(rm)))
+    pick = lambda t: int(np.nonzero(t <= t.min() * 1.015)[0][0])
+    assert be.variants() == (pick(cm), pick(rm))
     assert hipcore.default_variants(n) == be.variants()
     fresh = hip.Backend(amp, capacity=2)
     assert fresh.variants() == be.variants()
