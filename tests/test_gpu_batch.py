@@ -18,7 +18,8 @@ pytestmark = pytest.mark.gpu
 
 SIZES = [128, 384, 128, 256, 768, 384]
 POPS = [3, 2, 2, 2, 1, 2]
-SEQ = [("HIO", [5, 0.9]), ("SW", [2.0, 0.1]), ("RAAR", [3, 0.87]), ("SWA", [2.0, 900]), ("ER", [2])]
+SEQ = [("HIO", [5, 0.9]), ("SW", [2.0, 0.1]), ("RAAR", [3, 0.87]), ("GRAD", [3, 1e-2]), ("SWA", [2.0, 900]),
+       ("ER", [2])]
 
 
 def cases():
