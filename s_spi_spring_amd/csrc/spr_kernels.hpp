@@ -258,16 +258,9 @@ __device__ __forceinline__ float2 polar_of_zero(float2 F, float a) {
     return make_float2(-a, signbit(F.y) ? s * a : -s * a);
 }
 
-// Fourier modulus projection P_M (sub_intensities). The measured-pixel case
-// scales by a/|F| through the hardware reciprocal square root (within 2 ulp of
-// the reference's a*F/|F|) whenever |F|^2 is a normal float; tiny, zero and
-// bounded values take the exact formula.
-__device__ __forceinline__ float2 project_modulus(float2 F, float a) {
-    const float m2 = F.y * F.y + F.x * F.x;
-    if (a >= 0.f && m2 > 1e-36f) {
-        const float s = a * rsqrtf(m2);
-        return make_float2(F.x * s, F.y * s);
-    }
+// Exact P_M for the rare cases: measured pixels whose |F|^2 is not a normal
+// float (zero, tiny or NaN) and bounded pixels (a < -1.5).
+__device__ __forceinline__ float2 project_modulus_exact(float2 F, float a, float m2) {
     const float m = sqrtf(m2);
     if (a >= 0.f) {
         if (m > 0.f) return make_float2(a * F.x / m, a * F.y / m);
@@ -278,6 +271,20 @@ __device__ __forceinline__ float2 project_modulus(float2 F, float a) {
         return make_float2(nm * F.x / m, nm * F.y / m);
     }
     return F;
+}
+
+// Fourier modulus projection P_M (sub_intensities). The measured-pixel case
+// scales by a/|F| through the hardware reciprocal square root (within 2 ulp of
+// the reference's a*F/|F|) whenever |F|^2 is a normal float, and free pixels
+// (-1.5 <= a < 0) keep F (scale 1): both without a branch. Only a wavefront
+// holding a tiny, zero or bounded value runs the exact formula.
+__device__ __forceinline__ float2 project_modulus(float2 F, float a) {
+    const float m2 = F.y * F.y + F.x * F.x;
+    const bool fast = a >= 0.f && m2 > 1e-36f;
+    const float s = fast ? a * rsqrtf(m2) : 1.f;
+    float2 r = make_float2(F.x * s, F.y * s);
+    if (!fast && !(a < 0.f && a >= -1.5f)) r = project_modulus_exact(F, a, m2);
+    return r;
 }
 
 enum Alg { ALG_ER = 0, ALG_HIO = 1, ALG_RAAR = 2 };
