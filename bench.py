@@ -365,8 +365,9 @@ def main():
     be.timing(False)
     ind_iters = args.steps * args.block * P
     value = ind_iters / dt
-    col_b, row_b = algorithmic_bytes(n, P, args.alg)
-    iter_bytes = (col_b + row_b) / P
+    chunk_bytes, chunk = be.chunk()
+    col_b, row_b = algorithmic_bytes(n, min(P, chunk), args.alg)   # per launch (one chunk)
+    iter_bytes = (col_b + row_b) / min(P, chunk)
     iteration_frac = value * iter_bytes / 1e9 / HBM_PEAK_GBS
     log(f"whole iteration: {value * iter_bytes / 1e9:.0f} GB/s algorithmic ({iteration_frac:.3f} of HBM peak)")
     roofline = None
@@ -412,7 +413,8 @@ def main():
         "config": {"workload": f"ellipsoids {n}^2 pattern, population {P}, {args.alg}(beta={args.beta}) "
                                f"blocks of {args.block} iterations, population resident in HBM",
                    "n": n, "population": P, "alg": args.alg, "iterations_per_step": args.block,
-                   "kernel_variants": dict(zip(("column", "row"), be.variants()))},
+                   "kernel_variants": dict(zip(("column", "row"), be.variants())),
+                   "chunk_individuals": min(P, chunk)},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -286,6 +286,20 @@ int spr_stream(spr_ctx* ctx, void** stream);
  *   run on the target GPU, spr_tuned.hpp).
  * spr_batch_autotune: spr_autotune on every size class of a batch. */
 int spr_variants(int n, int* col_count, int* row_count);
+
+/* Chunked iteration blocks. No counterpart in the reference. An ER/HIO/RAAR
+ * block runs over the population in chunks whose densities, fields and
+ * supports (17 bytes per pixel) fit in `bytes`, each chunk through all of the
+ * block's iterations before the next, so that a chunk's working set stays in
+ * the last-level cache (the 256 MB Infinity Cache of MI355X) instead of every
+ * pass streaming the whole population from HBM. A population whose working
+ * set fits runs whole; a larger one runs in the fewest equal chunks that fit.
+ * The results are identical to an unchunked block. bytes = 0 runs the whole
+ * population at once. New handles start with the compiled default (300 MB) or
+ * SPR_CHUNK_MB from the environment; spr_get_chunk also returns the
+ * individuals per chunk. */
+int spr_set_chunk(spr_ctx* ctx, long long bytes);
+int spr_get_chunk(spr_ctx* ctx, long long* bytes, int* individuals);
 int spr_autotune(spr_ctx* ctx, int reps, float* col_ms, float* row_ms);
 int spr_set_variants(spr_ctx* ctx, int col, int row);
 int spr_get_variants(spr_ctx* ctx, int* col, int* row);
@@ -325,6 +339,8 @@ int spr_batch_run(spr_batch* b, const spr_op* ops, int nops, int nrep, float rea
 int spr_batch_evaluate(spr_batch* b, float reality, float* error_fourier, float* error_real, float* complexsum,
                        float* modulesum);
 int spr_batch_autotune(spr_batch* b, int reps);
+/* spr_set_chunk on every size class of a batch. */
+int spr_batch_set_chunk(spr_batch* b, long long bytes);
 
 #ifdef __cplusplus
 }

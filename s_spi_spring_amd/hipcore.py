@@ -71,7 +71,10 @@ SIGNATURES = {
     "spr_batch_run": (_I, [_VP, _VP, _I, _I, _F]),
     "spr_batch_evaluate": (_I, [_VP, _F, _VP, _VP, _VP, _VP]),
     "spr_batch_autotune": (_I, [_VP, _I]),
+    "spr_batch_set_chunk": (_I, [_VP, ctypes.c_longlong]),
     "spr_variants": (_I, [_I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "spr_set_chunk": (_I, [_VP, ctypes.c_longlong]),
+    "spr_get_chunk": (_I, [_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I)]),
     "spr_autotune": (_I, [_VP, _I, _VP, _VP]),
     "spr_set_variants": (_I, [_VP, _I, _I]),
     "spr_get_variants": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
@@ -353,6 +356,16 @@ class Backend:
         self._check(self.lib.spr_timing_read(self.h, int(kind), ctypes.byref(ms), ctypes.byref(cnt)))
         return ms.value, cnt.value
 
+    def set_chunk(self, nbytes):
+        """Working set per chunk of an iteration block (0: the whole population)."""
+        self._check(self.lib.spr_set_chunk(self.h, int(nbytes)))
+
+    def chunk(self):
+        """(bytes, individuals) per chunk of an iteration block."""
+        b, k = ctypes.c_longlong(), ctypes.c_int()
+        self._check(self.lib.spr_get_chunk(self.h, ctypes.byref(b), ctypes.byref(k)))
+        return b.value, k.value
+
     def variants(self):
         """(column variant, row variant) of this handle."""
         c, r = ctypes.c_int(), ctypes.c_int()
@@ -444,6 +457,10 @@ class Batch:
         ops = make_ops(sequence)
         self._check(self.lib.spr_batch_run(self.h, ctypes.cast(ops, ctypes.c_void_p), len(sequence), int(nrep),
                                            float(reality)))
+
+    def set_chunk(self, nbytes):
+        """spr_set_chunk on every size class (see Backend.set_chunk)."""
+        self._check(self.lib.spr_batch_set_chunk(self.h, int(nbytes)))
 
     def autotune(self, reps=5):
         """spr_autotune on every size class (see Backend.autotune)."""

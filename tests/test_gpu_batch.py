@@ -26,10 +26,15 @@ def cases():
     return [case(n, P, seed=20 + i) for i, (n, P) in enumerate(zip(SIZES, POPS))]
 
 
-def test_batch_equals_single_pattern_handles(hip):
+@pytest.mark.parametrize("chunk", [None, 1])
+def test_batch_equals_single_pattern_handles(hip, chunk):
+    """chunk = 1: iteration blocks in chunks of one individual, so that every
+    chunk after the first reads its amplitude through an offset pattern map."""
     cs = cases()
-    print("batch: sizes", SIZES, "populations", POPS, flush=True)
+    print("batch: sizes", SIZES, "populations", POPS, "chunk", chunk, flush=True)
     b = hipcore.Batch([c[0] for c in cs], POPS)
+    if chunk is not None:
+        b.set_chunk(1)
     for k, (amp, rho, supp) in enumerate(cs):
         b.set_population(k, rho, supp)
     b.run(SEQ, nrep=2, reality=0.0)

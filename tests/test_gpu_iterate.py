@@ -108,3 +108,22 @@ def test_sampled_timing_counts(hip):
     be.iterate("HIO", 2, 0.9, 0.0)
     assert be.timing_read(hip.KIND_ROW_UPDATE)[1] == 0
 
+
+
+@pytest.mark.parametrize("alg,beta", [("HIO", 0.9), ("RAAR", 0.87), ("ER", 0.0)])
+def test_chunked_blocks_are_bit_identical(hip, alg, beta):
+    """Chunked iteration blocks (2, 3 and 1 individuals per chunk, the last chunk
+    shorter) compute exactly what one block over the whole population does."""
+    n, P = 128, 7
+    amp, rho, supp = case(n, P, seed=8)
+    out = []
+    for per in (0, 2, 3, 1):
+        be = hip.Backend(amp, capacity=P)
+        be.set_chunk(per * 17 * n * n)
+        be.set_population(rho, supp)
+        assert be.chunk()[1] == (P if per == 0 else -(-P // -(-P // per)))
+        be.iterate(alg, 4, beta, 0.0)
+        out.append(be.get_population())
+        be.close()
+    for got in out[1:]:
+        assert np.array_equal(got[0], out[0][0]) and np.array_equal(got[1], out[0][1])
