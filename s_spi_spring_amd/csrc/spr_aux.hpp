@@ -849,19 +849,4 @@ __global__ __launch_bounds__(256) void k_copy_supports(const uint4* src, const i
     dst[(size_t)(didx ? didx[item] : item) * nw + p] = src[(size_t)(sidx ? sidx[item] : item) * nw + p];
 }
 
-// API supports (int32, nonzero = inside) <-> resident supports (uint8 0/1) of
-// the slots slot[k]; the int32 side is a dense staging array [k][np].
-__global__ __launch_bounds__(256) void k_supports_to_u8(const int* in, const int* slot, uint8_t* supp, int np) {
-    const int item = blockIdx.y;
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= np) return;
-    supp[(size_t)slot[item] * np + p] = in[(size_t)item * np + p] != 0 ? 1 : 0;
-}
-__global__ __launch_bounds__(256) void k_supports_to_i32(const uint8_t* supp, const int* slot, int* out, int np) {
-    const int item = blockIdx.y;
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= np) return;
-    out[(size_t)item * np + p] = supp[(size_t)slot[item] * np + p];
-}
-
 }  // namespace spr

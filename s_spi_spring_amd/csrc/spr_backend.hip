@@ -10,6 +10,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -64,7 +65,8 @@ struct spr_ctx {
         void* p = nullptr;
         size_t bytes = 0;
     };
-    Buf spec, xf, xs, supp2, fieldbuf, ampavg, xidx, shiftp, xchild, xmaps, xphase, adam_m, adam_v, gidx, sstage;
+    Buf spec, xf, xs, supp2, fieldbuf, ampavg, xidx, shiftp, xchild, xmaps, xphase, adam_m, adam_v, gidx;
+    Buf pinned;                   // pinned host staging of spr_put/fetch_individuals (hipHostMalloc)
     std::string err;
     // Opt-in per-launch timing: events bracket every launch on the stream.
     bool timing = false;
@@ -944,6 +946,34 @@ int autotune(spr_ctx* c, int reps, float* col_ms, float* row_ms) {
     return SPR_OK;
 }
 
+// Pinned host staging, grown on demand.
+int ensure_pinned(spr_ctx* c, size_t bytes) {
+    spr_ctx::Buf& b = c->pinned;
+    if (b.bytes >= bytes) return SPR_OK;
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    if (b.p) SPR_HIP(c, hipHostFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+    SPR_HIP(c, hipHostMalloc(&b.p, bytes, hipHostMallocDefault));
+    b.bytes = bytes;
+    return SPR_OK;
+}
+
+// f(i) for i in [0, count) on up to 16 host threads (host-side copies of whole fields).
+template <class F> void host_parallel(int count, F f) {
+    const int nt = std::min(count, std::max(1, std::min(16, (int)std::thread::hardware_concurrency())));
+    if (nt <= 1) {
+        for (int i = 0; i < count; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; ++t)
+        pool.emplace_back([&, t] {
+            for (int i = t; i < count; i += nt) f(i);
+        });
+    for (std::thread& th : pool) th.join();
+}
+
 int check_indices(spr_ctx* c, const int32_t* idx, int count, int limit, const char* what) {
     for (int i = 0; i < count; ++i)
         if (idx[i] < 0 || idx[i] >= limit) return fail(c, SPR_ERR_ARG, std::string(what) + ": index out of range");
@@ -1100,8 +1130,9 @@ int spr_destroy(spr_ctx* c) {
     (void)hipFree(c->radix);
     (void)hipFree(c->hist);
     for (spr_ctx::Buf* b : {&c->spec, &c->xf, &c->xs, &c->supp2, &c->fieldbuf, &c->ampavg, &c->xidx, &c->shiftp,
-                            &c->xchild, &c->xmaps, &c->xphase, &c->adam_m, &c->adam_v, &c->gidx, &c->sstage})
+                            &c->xchild, &c->xmaps, &c->xphase, &c->adam_m, &c->adam_v, &c->gidx})
         (void)hipFree(b->p);
+    if (c->pinned.p) (void)hipHostFree(c->pinned.p);
     if (c->lane2) (void)hipStreamSynchronize(c->lane2);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
@@ -1231,26 +1262,31 @@ int spr_put_individuals(spr_ctx* c, int count, const int* slots, const float* co
     if (!slots) return fail(c, SPR_ERR_ARG, "slots is NULL");
     for (int i = 0; i < count; ++i)
         if (slots[i] < 0 || slots[i] >= c->cap) return fail(c, SPR_ERR_ARG, "slot index out of range");
+    if (support)
+        for (int i = 0; i < count; ++i)
+            if (!support[i]) return fail(c, SPR_ERR_ARG, "support pointer is NULL");
     const size_t np = (size_t)c->n * c->n;
     SPR_HIP(c, hipSetDevice(c->device));
-    if (rho)
-        for (int i = 0; i < count; ++i)
-            if (rho[i])
-                SPR_HIP(c, hipMemcpyAsync(c->rho + (size_t)slots[i] * np, rho[i], np * sizeof(float2),
-                                          hipMemcpyHostToDevice, c->stream));
-    if (support) {
-        SPR_TRY(ensure(c, c->gidx, count * sizeof(int)));
-        SPR_TRY(ensure(c, c->sstage, (size_t)count * np * sizeof(int32_t)));
-        int32_t* stage = (int32_t*)c->sstage.p;
-        for (int i = 0; i < count; ++i) {
-            if (!support[i]) return fail(c, SPR_ERR_ARG, "support pointer is NULL");
-            SPR_HIP(c, hipMemcpyAsync(stage + (size_t)i * np, support[i], np * sizeof(int32_t),
-                                      hipMemcpyHostToDevice, c->stream));
+    // Host arrays -> pinned staging (densities as they are, supports narrowed to
+    // uint8 0/1) on host threads, then one DMA copy per individual and part.
+    const size_t fb = np * sizeof(float2);
+    SPR_TRY(ensure_pinned(c, (size_t)count * (fb + np)));
+    char* stage = (char*)c->pinned.p;
+    host_parallel(count, [&](int i) {
+        if (rho && rho[i]) std::memcpy(stage + (size_t)i * fb, rho[i], fb);
+        if (support) {
+            uint8_t* d = (uint8_t*)(stage + (size_t)count * fb + (size_t)i * np);
+            const int32_t* src = support[i];
+            for (size_t k = 0; k < np; ++k) d[k] = src[k] != 0 ? 1 : 0;
         }
-        SPR_HIP(c, hipMemcpyAsync(c->gidx.p, slots, count * sizeof(int), hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_supports_to_u8, dim3((unsigned)((np + 255) / 256), count), dim3(256), 0, c->stream,
-                           (const int*)stage, (const int*)c->gidx.p, c->supp, (int)np);
-        SPR_HIP(c, hipGetLastError());
+    });
+    for (int i = 0; i < count; ++i) {
+        if (rho && rho[i])
+            SPR_HIP(c, hipMemcpyAsync(c->rho + (size_t)slots[i] * np, stage + (size_t)i * fb, fb,
+                                      hipMemcpyHostToDevice, c->stream));
+        if (support)
+            SPR_HIP(c, hipMemcpyAsync(c->supp + (size_t)slots[i] * np, stage + (size_t)count * fb + (size_t)i * np,
+                                      np, hipMemcpyHostToDevice, c->stream));
     }
     SPR_HIP(c, hipStreamSynchronize(c->stream));
     return SPR_OK;
@@ -1265,25 +1301,28 @@ int spr_fetch_individuals(spr_ctx* c, int count, const int* slots, float* const*
         if (slots[i] < 0 || slots[i] >= c->cap) return fail(c, SPR_ERR_ARG, "slot index out of range");
     const size_t np = (size_t)c->n * c->n;
     SPR_HIP(c, hipSetDevice(c->device));
-    if (support) {
-        SPR_TRY(ensure(c, c->gidx, count * sizeof(int)));
-        SPR_TRY(ensure(c, c->sstage, (size_t)count * np * sizeof(int32_t)));
-        int32_t* stage = (int32_t*)c->sstage.p;
-        SPR_HIP(c, hipMemcpyAsync(c->gidx.p, slots, count * sizeof(int), hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_supports_to_i32, dim3((unsigned)((np + 255) / 256), count), dim3(256), 0, c->stream,
-                           (const uint8_t*)c->supp, (const int*)c->gidx.p, stage, (int)np);
-        SPR_HIP(c, hipGetLastError());
-        for (int i = 0; i < count; ++i)
-            if (support[i])
-                SPR_HIP(c, hipMemcpyAsync(support[i], stage + (size_t)i * np, np * sizeof(int32_t),
-                                          hipMemcpyDeviceToHost, c->stream));
+    // One DMA copy per individual and part into pinned staging (supports as
+    // uint8), then host threads copy the densities out and widen the supports.
+    const size_t fb = np * sizeof(float2);
+    SPR_TRY(ensure_pinned(c, (size_t)count * (fb + np)));
+    char* stage = (char*)c->pinned.p;
+    for (int i = 0; i < count; ++i) {
+        if (rho && rho[i])
+            SPR_HIP(c, hipMemcpyAsync(stage + (size_t)i * fb, c->rho + (size_t)slots[i] * np, fb,
+                                      hipMemcpyDeviceToHost, c->stream));
+        if (support && support[i])
+            SPR_HIP(c, hipMemcpyAsync(stage + (size_t)count * fb + (size_t)i * np, c->supp + (size_t)slots[i] * np,
+                                      np, hipMemcpyDeviceToHost, c->stream));
     }
-    if (rho)
-        for (int i = 0; i < count; ++i)
-            if (rho[i])
-                SPR_HIP(c, hipMemcpyAsync(rho[i], c->rho + (size_t)slots[i] * np, np * sizeof(float2),
-                                          hipMemcpyDeviceToHost, c->stream));
     SPR_HIP(c, hipStreamSynchronize(c->stream));
+    host_parallel(count, [&](int i) {
+        if (rho && rho[i]) std::memcpy(rho[i], stage + (size_t)i * fb, fb);
+        if (support && support[i]) {
+            const uint8_t* src = (const uint8_t*)(stage + (size_t)count * fb + (size_t)i * np);
+            int32_t* d = support[i];
+            for (size_t k = 0; k < np; ++k) d[k] = src[k];
+        }
+    });
     return SPR_OK;
 }
 
