@@ -465,17 +465,21 @@ RecVec Solver::Crossover(RecVec rec, RecPtr best, RecPtr avg, FLOAT crossbalance
         }
     }
     place(rec);
-    // Child buffers are written completely by spr_crossover: no initialisation.
-    std::unique_ptr<float[]> child(new float[(std::size_t)P * size * 2]);
-    std::unique_ptr<SUPPORTTYPE[]> child_s(new SUPPORTTYPE[(std::size_t)P * size]);
+    if (child_cap < (std::size_t)P) {
+        child_rho.reset(new float[(std::size_t)P * size * 2]);
+        child_supp.reset(new SUPPORTTYPE[(std::size_t)P * size]);
+        child_cap = (std::size_t)P;
+    }
+    float* child = child_rho.get();
+    SUPPORTTYPE* child_s = child_supp.get();
     check(spr_crossover(ctx, P, desc.data(), maps.data(), (int)maps.size(),
                         reinterpret_cast<const float*>(avg->density.data()), avg_mod.data(), crossdiff, use_average,
-                        shiftmin, shiftmax, reality, update_support ? 1 : 0, child.get(), child_s.get()));
+                        shiftmin, shiftmax, reality, update_support ? 1 : 0, child, child_s));
     // Each child starts as the reference's copy of rec[i] (field values, error and
     // error_real; src/reconstruction.h:35-45) with the crossed density (and support).
     parallel_for(P, [&](int i) {
-        const COMPLEX* d = reinterpret_cast<const COMPLEX*>(child.get()) + (std::size_t)i * size;
-        const SUPPORTTYPE* sp = update_support ? child_s.get() + (std::size_t)i * size : rec[i]->support.data();
+        const COMPLEX* d = reinterpret_cast<const COMPLEX*>(child) + (std::size_t)i * size;
+        const SUPPORTTYPE* sp = update_support ? child_s + (std::size_t)i * size : rec[i]->support.data();
         RecPtr son = std::make_shared<Reconstruction>(d, sp, dims);
         son->error = rec[i]->error;
         son->error_real = rec[i]->error_real;

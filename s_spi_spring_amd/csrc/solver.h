@@ -107,6 +107,11 @@ private:
     spr_ctx* ctx = nullptr;
     int ctx_capacity = 0;
     std::vector<Slot> slots;
+    // Crossover output buffers, kept between generations (written completely by
+    // every spr_crossover, so never initialised).
+    std::unique_ptr<float[]> child_rho;
+    std::unique_ptr<SUPPORTTYPE[]> child_supp;
+    std::size_t child_cap = 0;   // individuals
 };
 
 }  // namespace spring

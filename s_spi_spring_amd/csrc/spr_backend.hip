@@ -1738,15 +1738,24 @@ int spr_crossover(spr_ctx* c, int nchild, const spr_cross_desc* desc, const int3
     hipLaunchKernelGGL(k_er_project, gp, dim3(256), 0, c->stream, xf, (const int*)d_outf, (const uint8_t*)xs,
                        (const int*)d_outs, (int)np, pc);
     SPR_HIP(c, hipGetLastError());
+    // Children (field 5k, support 3k) through pinned staging; host threads copy
+    // the fields out and widen the supports.
+    const size_t fb = np * sizeof(float2);
+    SPR_TRY(ensure_pinned(c, (size_t)K * (fb + np)));
+    char* stage = (char*)c->pinned.p;
     for (int k = 0; k < K; ++k) {
-        SPR_HIP(c, hipMemcpyAsync(child_rho + 2 * (size_t)k * np, xf + (size_t)(5 * k) * np, np * sizeof(float2),
+        SPR_HIP(c, hipMemcpyAsync(stage + (size_t)k * fb, xf + (size_t)(5 * k) * np, fb, hipMemcpyDeviceToHost,
+                                  c->stream));
+        SPR_HIP(c, hipMemcpyAsync(stage + (size_t)K * fb + (size_t)k * np, xs + (size_t)(3 * k) * np, np,
                                   hipMemcpyDeviceToHost, c->stream));
     }
-    std::vector<uint8_t> hs(3 * (size_t)K * np);
-    SPR_HIP(c, hipMemcpyAsync(hs.data(), xs, hs.size(), hipMemcpyDeviceToHost, c->stream));
     SPR_HIP(c, hipStreamSynchronize(c->stream));
-    for (int k = 0; k < K; ++k)
-        for (size_t i = 0; i < np; ++i) child_supp[(size_t)k * np + i] = hs[(size_t)(3 * k) * np + i];
+    host_parallel(K, [&](int k) {
+        std::memcpy(child_rho + 2 * (size_t)k * np, stage + (size_t)k * fb, fb);
+        const uint8_t* src = (const uint8_t*)(stage + (size_t)K * fb + (size_t)k * np);
+        int32_t* d = child_supp + (size_t)k * np;
+        for (size_t i = 0; i < np; ++i) d[i] = src[i];
+    });
     return harvest_timing(c);
 }
 
