@@ -32,7 +32,9 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "individual-iters/s at 512² pop64; time-to-Rf<0.05, batched+grad vs plain GPU"
 ROW_MODE = {"HIO": 2, "RAAR": 3, "ER": 4}       # RowMode of spr_kernels.hpp
 TIMING_STRIDE = 8
-DEVICE_SOURCES = ("spr_fft.hpp", "spr_kernels.hpp", "spr_aux.hpp", "spr_grad.hpp", "spr_tuned.hpp", "spr_backend.hip")
+DEVICE_SOURCES = ("spr_fft.hpp", "spr_kernels.hpp", "spr_fused.hpp", "spr_aux.hpp", "spr_grad.hpp", "spr_tuned.hpp",
+                  "spr_backend.hip")
+FP32_PEAK_TFLOPS = 157.3
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_iteration_512_pop64_raar.json")
 COUNTER_CMD = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 90 rocprofv3 --pmc {ctr} --kernel-trace "
                "--output-format csv -d <dir> -o run -- python3 -u /root/repo/bench.py --steps 2 --warmup 1 "
@@ -107,7 +109,16 @@ def device_source_hash():
     return h.hexdigest()[:16]
 
 
+def fused_flops(n, P, block):
+    """Nominal FFT flops per launch of the on-chip block kernel: 4(block+1) sets
+    of N length-N transforms (5 N log2 N flops each) per individual; the
+    projection and update arithmetic are not counted."""
+    return 4 * (block + 1) * 5 * n * n * np.log2(n) * P
+
+
 def rocprof_kernel_name(kernel, n, alg, variant):
+    if kernel == "fused":
+        return f"void spr::k_iter_fused<{n}, {ROW_MODE[alg]}>(spr::FusedArgs)"
     if kernel == "row":
         return f"void spr::k_row<{n}, {ROW_MODE[alg]}, {variant}>(spr::RowArgs)"
     return f"void spr::k_col<{n}, 0, {variant}>(spr::ColArgs)"
@@ -362,16 +373,29 @@ def main():
     dt = time.perf_counter() - t0
     col_ms, col_cnt = be.timing_read(hipcore.KIND_COL_PROJ)
     row_ms, row_cnt = be.timing_read(hipcore.KIND_ROW_UPDATE)
+    blk_ms, blk_cnt = be.timing_read(hipcore.KIND_BLOCK)
     be.timing(False)
     ind_iters = args.steps * args.block * P
     value = ind_iters / dt
     chunk_bytes, chunk = be.chunk()
+    fused = be.fused()[1]
     col_b, row_b = algorithmic_bytes(n, min(P, chunk), args.alg)   # per launch (one chunk)
     iter_bytes = (col_b + row_b) / min(P, chunk)
     iteration_frac = value * iter_bytes / 1e9 / HBM_PEAK_GBS
-    log(f"whole iteration: {value * iter_bytes / 1e9:.0f} GB/s algorithmic ({iteration_frac:.3f} of HBM peak)")
+    if not fused:
+        log(f"whole iteration: {value * iter_bytes / 1e9:.0f} GB/s algorithmic ({iteration_frac:.3f} of HBM peak)")
     roofline = None
-    if col_cnt and row_cnt:
+    if fused and blk_cnt:
+        # One launch per block, the field in LDS: bounded by the vector ALUs.
+        avg = blk_ms / blk_cnt * 1e-3
+        tf = fused_flops(n, P, args.block) / avg / 1e12
+        log(f"on-chip block: {blk_cnt} launches, avg {avg * 1e6:.1f} us, {tf:.1f} TFLOP/s nominal FFT flops")
+        roofline = {"bound": "valu", "achieved": round(tf, 2), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(tf / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                    "kernel": rocprof_kernel_name("fused", n, args.alg, 0),
+                    "algorithmic_flops_per_launch": fused_flops(n, P, args.block),
+                    "avg_launch_us": round(avg * 1e6, 2)}
+    elif col_cnt and row_cnt:
         col_avg = col_ms / col_cnt * 1e-3
         row_avg = row_ms / row_cnt * 1e-3
         col_gbs = col_b / col_avg / 1e9
@@ -410,11 +434,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"ellipsoids {n}^2 pattern, population {P}, {args.alg}(beta={args.beta}) "
+        "config": {"workload": f"{'ellipsoids' if n >= 256 else 'polygon'} {n}^2 pattern, population {P}, {args.alg}(beta={args.beta}) "
                                f"blocks of {args.block} iterations, population resident in HBM",
                    "n": n, "population": P, "alg": args.alg, "iterations_per_step": args.block,
                    "kernel_variants": dict(zip(("column", "row"), be.variants())),
-                   "chunk_individuals": min(P, chunk)},
+                   "chunk_individuals": min(P, chunk), "on_chip_blocks": fused},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -252,7 +252,8 @@ enum spr_timing_kind {
     SPR_KIND_ROW_UPDATE = 1,   /* row pass rowIFFT -> HIO/RAAR/ER update -> rowFFT */
     SPR_KIND_ROW_OTHER = 2,    /* other row passes (first/last of an operation) */
     SPR_KIND_COL_OTHER = 3,    /* other column passes */
-    SPR_TIMING_KINDS = 4
+    SPR_KIND_BLOCK = 4,        /* whole iteration blocks on chip (64^2, 128^2; spr_set_fused) */
+    SPR_TIMING_KINDS = 5
 };
 int spr_timing(spr_ctx* ctx, int enable);
 int spr_timing_read(spr_ctx* ctx, int kind, double* total_ms, long* count);
@@ -300,6 +301,16 @@ int spr_variants(int n, int* col_count, int* row_count);
  * individuals per chunk. */
 int spr_set_chunk(spr_ctx* ctx, long long bytes);
 int spr_get_chunk(spr_ctx* ctx, long long* bytes, int* individuals);
+/* On-chip iteration blocks. No counterpart in the reference. For 64^2 and
+ * 128^2 fields an ER/HIO/RAAR block can run as one launch with one workgroup
+ * per individual holding the whole field in LDS, so that HBM is read at the
+ * block's start and written at its end only. mode -1 (the default) uses it from
+ * a population of 8 individuals, 0 never, 1 always (SPR_ERR_ARG for other
+ * sizes). The results agree with the two-pass form within float rounding (the
+ * transforms use other radix plans). spr_get_fused returns the mode and whether
+ * the handle's next block would run on chip. */
+int spr_set_fused(spr_ctx* ctx, int mode);
+int spr_get_fused(spr_ctx* ctx, int* mode, int* active);
 int spr_autotune(spr_ctx* ctx, int reps, float* col_ms, float* row_ms);
 int spr_set_variants(spr_ctx* ctx, int col, int row);
 int spr_get_variants(spr_ctx* ctx, int* col, int* row);

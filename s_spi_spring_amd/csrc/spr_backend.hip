@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "spring_hip.h"
+#include "spr_fused.hpp"
 #include "spr_grad.hpp"
 #include "spr_tuned.hpp"
 
@@ -25,6 +26,16 @@
 // population that already fits (64 individuals, 285 MB) costs 8 %.
 #ifndef SPR_CHUNK_MB_DEFAULT
 #define SPR_CHUNK_MB_DEFAULT 300
+#endif
+
+// Populations from which iteration blocks of 64^2 and 128^2 fields run on chip
+// (one workgroup per individual) unless spr_set_fused says otherwise. Measured
+// on MI355X, RAAR blocks of 25 iterations: at 128^2 the on-chip form runs at
+// 1.03x the two-pass form with 8 individuals, 1.64x with 64 and 2.7x with 256;
+// at 64^2 at 1.27x with 8 and 2.5x with 256. Smaller populations, which occupy
+// fewer CUs than the two-pass form spreads them over, were not measured.
+#ifndef SPR_FUSED_MIN_P
+#define SPR_FUSED_MIN_P 8
 #endif
 
 using namespace spr;
@@ -50,6 +61,7 @@ struct spr_ctx {
     int colvar = 0, rowvar = 0;   // variants of the column and row iteration passes
     float2* twtab[MAX_VARIANTS] = {};  // global twiddle tables of the column variants that read one
     size_t chunk_bytes = 0;       // iteration blocks run in chunks of this working set (0: whole population)
+    int fused = -1;               // on-chip iteration blocks (N <= 128): -1 by population size, 0 never, 1 always
     int* amp_of = nullptr;        // slot -> pattern when npat > 1, else NULL
     float2* tw = nullptr;
     // reduction scratch
@@ -623,6 +635,40 @@ int queue_iterate_block(spr_ctx* c, int mode, int niter, const RowArgs& a0) {
     return SPR_OK;
 }
 
+// One iteration block with each individual's whole field in one workgroup's
+// LDS (spr_fused.hpp): one launch, HBM read at its start and written at its end.
+template <int N> int launch_fused(spr_ctx* c, int mode, int niter, const RowArgs& a) {
+    if constexpr (fused_size<N>()) {
+        const FusedArgs f{a.rho, a.supp, c->amp_t, c->amp_of, c->tw, niter, a.beta, a.inv_nn, a.pc};
+        const dim3 grid((unsigned)c->P), block(N * (N / 16));
+        switch (mode) {
+            case ROW_ER: hipLaunchKernelGGL((k_iter_fused<N, ROW_ER>), grid, block, 0, c->stream, f); break;
+            case ROW_HIO: hipLaunchKernelGGL((k_iter_fused<N, ROW_HIO>), grid, block, 0, c->stream, f); break;
+            case ROW_RAAR: hipLaunchKernelGGL((k_iter_fused<N, ROW_RAAR>), grid, block, 0, c->stream, f); break;
+            default: return fail(c, SPR_ERR_ARG, "bad iteration mode");
+        }
+        SPR_HIP(c, hipGetLastError());
+        return SPR_OK;
+    } else {
+        return fail(c, SPR_ERR_ARG, "no on-chip iteration block for this grid size");
+    }
+}
+
+int fused_available(int n) { return n == 64 || n == 128; }
+
+bool use_fused(const spr_ctx* c) {
+    if (!fused_available(c->n) || c->fused == 0) return false;
+    return c->fused == 1 || c->P >= SPR_FUSED_MIN_P;
+}
+
+int fused_n(spr_ctx* c, int mode, int niter, const RowArgs& a) { SPR_DISPATCH_N(c, launch_fused, c, mode, niter, a); }
+
+int queue_iterate_fused(spr_ctx* c, int mode, int niter, const RowArgs& a) {
+    SPR_TRY(timing_begin(c, SPR_KIND_BLOCK));
+    SPR_TRY(fused_n(c, mode, niter, a));
+    return timing_end(c);
+}
+
 int queue_iterate(spr_ctx* c, int alg, int niter, float beta, float reality) {
     if (alg < SPR_ER || alg > SPR_RAAR) return fail(c, SPR_ERR_ARG, "unknown algorithm");
     if (niter < 0) return fail(c, SPR_ERR_ARG, "negative iteration count");
@@ -632,6 +678,7 @@ int queue_iterate(spr_ctx* c, int alg, int niter, float beta, float reality) {
     a.pc = (alg == SPR_RAAR) ? cut_raar(reality) : cut_er_hio(reality);
     const int mode = (alg == SPR_HIO) ? ROW_HIO : (alg == SPR_RAAR) ? ROW_RAAR : ROW_ER;
     SPR_HIP(c, hipSetDevice(c->device));
+    if (use_fused(c)) return queue_iterate_fused(c, mode, niter, a);
     if (c->lanes > 1 && c->P >= 2 && !c->timing) return queue_iterate_lanes(c, mode, niter, a);
     return queue_iterate_block(c, mode, niter, a);
 }
@@ -1923,6 +1970,22 @@ int spr_get_chunk(spr_ctx* c, long long* bytes, int* individuals) {
     if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
     if (bytes) *bytes = (long long)c->chunk_bytes;
     if (individuals) *individuals = chunk_individuals(c);
+    return SPR_OK;
+}
+
+int spr_set_fused(spr_ctx* c, int mode) {
+    if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
+    if (mode < -1 || mode > 1) return fail(c, SPR_ERR_ARG, "fused mode must be -1, 0 or 1");
+    if (mode == 1 && !fused_available(c->n)) return fail(c, SPR_ERR_ARG, "no on-chip iteration block for this grid size");
+    SPR_HIP(c, hipStreamSynchronize(c->stream));
+    c->fused = mode;
+    return SPR_OK;
+}
+
+int spr_get_fused(spr_ctx* c, int* mode, int* active) {
+    if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
+    if (mode) *mode = c->fused;
+    if (active) *active = use_fused(c) ? 1 : 0;
     return SPR_OK;
 }
 

@@ -46,6 +46,8 @@ SIGNATURES = {
     "spr_device_buffers": (_I, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP), ctypes.POINTER(_VP)]),
     "spr_stream": (_I, [_VP, ctypes.POINTER(_VP)]),
     "spr_timing": (_I, [_VP, _I]),
+    "spr_set_fused": (_I, [_VP, _I]),
+    "spr_get_fused": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "spr_timing_read": (_I, [_VP, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]),
     "spr_sw": (_I, [_VP, _F, _F, _F]),
     "spr_swa": (_I, [_VP, _F, _I, _F]),
@@ -119,7 +121,7 @@ def make_ops(sequence):
             o.sigma, o.pixels = float(p[0]), int(p[1])
     return ops
 
-KIND_COL_PROJ, KIND_ROW_UPDATE, KIND_ROW_OTHER, KIND_COL_OTHER = 0, 1, 2, 3
+KIND_COL_PROJ, KIND_ROW_UPDATE, KIND_ROW_OTHER, KIND_COL_OTHER, KIND_BLOCK = 0, 1, 2, 3, 4
 
 
 class SpringHipError(RuntimeError):
@@ -365,6 +367,17 @@ class Backend:
         b, k = ctypes.c_longlong(), ctypes.c_int()
         self._check(self.lib.spr_get_chunk(self.h, ctypes.byref(b), ctypes.byref(k)))
         return b.value, k.value
+
+    def set_fused(self, mode):
+        """On-chip iteration blocks for 64^2 and 128^2 fields: -1 from a
+        population of 8 (default), 0 never, 1 always."""
+        self._check(self.lib.spr_set_fused(self.h, int(mode)))
+
+    def fused(self):
+        """(mode, whether the next iteration block runs on chip)."""
+        m, a = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.spr_get_fused(self.h, ctypes.byref(m), ctypes.byref(a)))
+        return m.value, bool(a.value)
 
     def variants(self):
         """(column variant, row variant) of this handle."""
