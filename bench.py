@@ -185,18 +185,23 @@ def measured_traffic(kernel_name, log):
     return None if k is None else k["hbm_bytes"]
 
 
-def time_to_rf_study(seeds, generations, target, log, threshold=None):
+def time_to_rf_study(seeds, generations, target, log, threshold=None, groups=()):
     """Time to best E_F < target of the plain and the gradient schedules over
-    `seeds` seeded 512^2 ellipsoid patterns (population 64); one JSON line."""
+    `seeds` seeded 512^2 ellipsoid patterns (population 64); with `groups`, also
+    the plain schedule run k individuals per launch for each k (the reference's
+    structure: each of its k GPU contexts runs one individual at a time). One
+    JSON line."""
     from s_spi_spring_amd import workloads
     log("warm-up run (not recorded)")
     workloads.time_to_rf(10_000, "plain", target=target, generations=0)
-    runs = {"plain": [], "grad": []}
+    legs = [("plain", "plain", 0), ("grad", "grad", 0)] + [(f"plain_groups_of_{k}", "plain", k) for k in groups]
+    runs = {name: [] for name, _, _ in legs}
     for seed in range(seeds):
-        for form in ("plain", "grad"):
-            r = workloads.time_to_rf(seed, form, target=target, generations=generations, threshold=threshold)
-            runs[form].append(r)
-            log(f"seed {seed:2d} {form:5s}: t={r['time_to_target']} s gen={r['generation']} "
+        for name, form, k in legs:
+            r = workloads.time_to_rf(seed, form, target=target, generations=generations, threshold=threshold,
+                                     max_batch=k)
+            runs[name].append(r)
+            log(f"seed {seed:2d} {name:5s}: t={r['time_to_target']} s gen={r['generation']} "
                 f"E_F={r['fourier_error']:.4f} (floor {r['noise_floor']:.4f}) real-space={r['real_space_error']:.4f} "
                 f"wall={r['wall']:.1f} s history={r['history']}")
 
@@ -212,11 +217,20 @@ def time_to_rf_study(seeds, generations, target, log, threshold=None):
                                              for q in (25, 75)]}
     sp, sg = summary(runs["plain"]), summary(runs["grad"])
     speedup = (sp["median_time_s"] / sg["median_time_s"]) if sp["median_time_s"] and sg["median_time_s"] else None
+    grouped = {}
+    for name, _, k in legs[2:]:
+        sk = summary(runs[name])
+        sk["individuals_per_launch"] = k
+        sk["speedup_grad_vs_this"] = (sk["median_time_s"] / sg["median_time_s"]
+                                      if sk["median_time_s"] and sg["median_time_s"] else None)
+        sk["speedup_plain_vs_this"] = (sk["median_time_s"] / sp["median_time_s"]
+                                       if sk["median_time_s"] and sp["median_time_s"] else None)
+        grouped[name] = sk
     out = {"metric": f"time-to-Rf<{target}", "unit": "s", "higher_is_better": False, "seeds": seeds,
            "generations_max": generations, "config": {"workload": "ellipsoids 512^2 pattern, 1e9 photons, "
                                                       "population 64, RAAR400+HIO100+ER50 per generation in "
                                                       "25-iteration blocks with shrink-wrap"},
-           "plain": sp, "grad": sg, "speedup_grad_vs_plain": speedup,
+           "plain": sp, "grad": sg, "speedup_grad_vs_plain": speedup, **grouped,
            "sw_threshold": threshold if threshold is not None else workloads.SW_THRESHOLD_512,
            "median_noise_floor": float(np.median([r["noise_floor"] for r in runs["plain"]])),
            "per_seed": {f: [{k: r[k] for k in ("seed", "time_to_target", "generation", "fourier_error",
@@ -307,6 +321,8 @@ def main():
                     help="instead of the throughput step: time to best E_F < --target of the plain and "
                          "gradient schedules over SEEDS patterns (one JSON line)")
     ap.add_argument("--target", type=float, default=0.05)
+    ap.add_argument("--plain-groups", default="", metavar="K,K",
+                    help="time-to-rf: also run the plain schedule K individuals per launch for each K")
     ap.add_argument("--generations", type=int, default=10)
     ap.add_argument("--sw-threshold", type=float, default=None,
                     help="shrink-wrap threshold of the time-to-R_F schedules (default 0.03)")
@@ -335,7 +351,8 @@ def main():
         sys.exit(2)
 
     if args.time_to_rf:
-        time_to_rf_study(args.time_to_rf, args.generations, args.target, log, args.sw_threshold)
+        time_to_rf_study(args.time_to_rf, args.generations, args.target, log, args.sw_threshold,
+                         groups=[int(k) for k in args.plain_groups.split(",") if k])
         return
 
     if args.batch:

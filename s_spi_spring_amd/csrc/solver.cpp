@@ -261,12 +261,27 @@ void Solver::pull(const RecVec& rec, bool density, bool support) {
             if (rec[i] == rec[j]) slots[i] = Slot{};
 }
 
+// The whole population, or groups of max_batch individuals each copied in
+// from the host (no device-resident copy is reused) and back.
+void Solver::for_groups(const RecVec& rec, const std::function<void(const RecVec&)>& f) {
+    if (max_batch <= 0 || (int)rec.size() <= max_batch) {
+        f(rec);
+        return;
+    }
+    for (std::size_t i = 0; i < rec.size(); i += (std::size_t)max_batch) {
+        forget((int)slots.size());
+        f(RecVec(rec.begin() + i, rec.begin() + std::min(rec.size(), i + (std::size_t)max_batch)));
+    }
+}
+
 void Solver::run_op(const RecVec& rec, const spr_op& op) {
     if (rec.empty()) return;
-    place(rec);
-    check(spr_run(ctx, &op, 1, 1, reality));
     const bool support_only = op.kind == SPR_OP_SW || op.kind == SPR_OP_SWA;
-    pull(rec, !support_only, true);
+    for_groups(rec, [&](const RecVec& g) {
+        place(g);
+        check(spr_run(ctx, &op, 1, 1, reality));
+        pull(g, !support_only, true);
+    });
 }
 
 // Solver::HIO / RAAR / ER / SW / SWA (src/solver.cpp:360-450).
@@ -342,10 +357,12 @@ RecVec Solver::Sequence(RecVec rec, int nrep) {
         return rec;
     }
     if (rec.empty()) return rec;
-    place(rec);
-    check(spr_run(ctx, ops.data(), (int)ops.size(), nrep, reality));
-    evaluate_resident(rec);
-    pull(rec, true, true);
+    for_groups(rec, [&](const RecVec& g) {
+        place(g);
+        check(spr_run(ctx, ops.data(), (int)ops.size(), nrep, reality));
+        evaluate_resident(g);
+        pull(g, true, true);
+    });
     return rec;
 }
 

@@ -60,6 +60,13 @@ public:
     void set_pattern(const FLOAT* pattern_in);
     void set_sequence(const std::vector<SeqItem>& sequence);
     void set_reality(FLOAT r) { reality = r; }
+    // Individuals per device launch of Sequence and HIO/RAAR/ER/SW/SWA; 0 (the
+    // default) runs the whole population at once. With k > 0 the population
+    // runs k individuals at a time, each group copied in from the host before
+    // its operations and back after them, the way src/solver.cpp:360-487 hands
+    // one individual at a time to each of its GPU contexts.
+    void set_max_batch(int k) { max_batch = k < 0 ? 0 : k; }
+    int get_max_batch() const { return max_batch; }
     void set_shift_param(FLOAT minr, FLOAT maxr) {
         shiftmin = minr;
         shiftmax = maxr;
@@ -112,6 +119,8 @@ private:
     std::unique_ptr<float[]> child_rho;
     std::unique_ptr<SUPPORTTYPE[]> child_supp;
     std::size_t child_cap = 0;   // individuals
+    int max_batch = 0;
+    void for_groups(const RecVec& rec, const std::function<void(const RecVec&)>& f);
 };
 
 }  // namespace spring

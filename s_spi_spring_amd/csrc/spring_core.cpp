@@ -154,6 +154,8 @@ PYBIND11_MODULE(spring_core, sp) {
         .def("Evaluate", [](Solver& s, OneOrMany rec) { s.Evaluate(as_vec(rec)); })
         .def("set_sequence", [](Solver& s, std::vector<SeqItem> seq) { s.set_sequence(seq); })
         .def("set_reality", [](Solver& s, FLOAT r) { s.set_reality(r); })
+        .def("set_max_batch", [](Solver& s, int k) { s.set_max_batch(k); }, py::arg("k"))
+        .def("get_max_batch", [](Solver& s) { return s.get_max_batch(); })
         .def("InitializePop",
              [](Solver& s, int nrec, int support_size, std::array<FLOAT, 2> itemsize, std::array<int, 2> nitems,
                 std::array<FLOAT, 2> exponent, std::array<FLOAT, 2> scaling, FLOAT phase, FLOAT suppcut) {

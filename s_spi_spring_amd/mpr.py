@@ -60,6 +60,10 @@ class MPR:
         self.stopevent = None
         self.logqueue = None
         self.history = []
+        # Individuals per device launch of the Solver's sequences (0: the whole
+        # population; k: groups of k, as the reference's one-individual-per-
+        # context GPU path with k contexts in flight).
+        self.max_batch = 0
 
     def print(self, msg, end="\n", flush=True):
         if self.logqueue is None:
@@ -83,6 +87,8 @@ class MPR:
                              nthreads=self.settings["global"]["threads"] * max(1, len(self.gpulist)),
                              seed=self.settings_adv["seed"])
         self.solver.set_shift_param(shiftmin=self.settings_adv["shift_min"], shiftmax=self.settings_adv["shift_max"])
+        if self.max_batch:
+            self.solver.set_max_batch(int(self.max_batch))
 
     def InitIA(self):
         ia, adv = self.settings["IA"], self.settings_adv

@@ -95,12 +95,13 @@ def noise_floor(I, truth):
 
 
 def time_to_rf(seed, form="plain", target=0.05, generations=10, popsize=64, n=512, photons=1e9, threshold=None,
-               schedule=None):
+               schedule=None, max_batch=0):
     """One memetic run on the ellipsoid pattern of `seed`; returns the time to
     best E_F < target (None when not reached within `generations`), the
     generation reached, the final Fourier and real-space errors of the best
     individual, the best E_F after every generation and the pattern's noise
-    floor."""
+    floor. max_batch > 0 runs the sequences in groups of that many individuals
+    (Solver.set_max_batch)."""
     from . import Pattern, Settings, metrics, synth
     from .mpr import MPR
     I, truth = synth.ellipsoids_pattern(seed=seed, n=n, photons=photons)
@@ -109,6 +110,7 @@ def time_to_rf(seed, form="plain", target=0.05, generations=10, popsize=64, n=51
     s.set("init", "supportsize", support_size(truth)).set("IA", "reality", 0.0)
     s.set_advanced("seed", 1000 + seed)
     m = MPR(Pattern.from_intensity(I), s)
+    m.max_batch = max_batch
     t0 = time.perf_counter()
     sched = SCHEDULES[form] if schedule is None else schedule
     res = m.run(target_error=target, log=False,
@@ -116,7 +118,7 @@ def time_to_rf(seed, form="plain", target=0.05, generations=10, popsize=64, n=51
     wall = time.perf_counter() - t0
     h = m.history[-1]
     reached = h["best_fourier_error"] < target
-    return {"seed": seed, "form": form, "time_to_target": h["time"] if reached else None,
+    return {"seed": seed, "form": form, "max_batch": max_batch, "time_to_target": h["time"] if reached else None,
             "wall": wall, "generation": h["generation"], "fourier_error": h["best_fourier_error"],
             "real_space_error": metrics.real_space_error(res.get_density_best(crop=False), truth),
             "history": [round(x["best_fourier_error"], 5) for x in m.history],

@@ -106,3 +106,31 @@ def test_resident_solver_matches_fresh_solvers(pr):
     assert _same(_state(A), _state(A0))
     assert _same(_state(B), _state(B0))
     assert np.array_equal(avg.get_density(), avg0.get_density()) and avg.get_error() == avg0.get_error()
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_groups_of_k_match_whole_population(pr, k):
+    """Solver.set_max_batch(k): sequences and single operations run k
+    individuals per launch, each group copied in from the host and back; every
+    individual's arithmetic is the same, so the results are bit-identical to
+    the whole population in one launch (256^2: the two-pass kernels at every
+    group size)."""
+    n, P = 256, 5
+    I, _ = synth.ellipsoids_pattern(seed=4, n=n)
+
+    def solver():
+        return pr.Solver(I, reality=0.0, realpen=0.0, nthreads=P, use_bounds=False, gpus=[0], seed=17)
+
+    S = solver()
+    A = _pop(pr, S, P, n)
+    B = _clone(pr, A)
+    seq = [("RAAR", [4.0, 0.87]), ("SW", [2.0, 0.1]), ("HIO", [3.0, 0.9]), ("ER", [2.0])]
+    whole, grouped = solver(), solver()
+    grouped.set_max_batch(k)
+    assert grouped.get_max_batch() == k and whole.get_max_batch() == 0
+    for s, pop in ((whole, A), (grouped, B)):
+        s.set_sequence(seq)
+        s.Sequence(pop, 2)
+        s.SWA(pop, 2.0, 900)
+        s.ER(pop, 2)
+    assert _same(_state(A), _state(B))
