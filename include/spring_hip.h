@@ -275,12 +275,14 @@ int spr_stream(spr_ctx* ctx, void** stream);
  * accuracy bar against the oracle.
  *
  * spr_variants: number of column and row variants of size n.
- * spr_autotune: times `reps` launches of every variant on the handle's
- *   resident population (densities and supports are left unchanged), keeps
- *   the fastest pair (per pass, the lowest-numbered variant within 1.5 % of
- *   the fastest) for this handle and makes it the default of handles of
- *   that size created later in the process; col_ms / row_ms (NULL or one
- *   entry per variant) receive the mean launch durations in ms.
+ * spr_autotune: times `reps` iterations (a column pass, then a RAAR row
+ *   pass) with every variant on the handle's resident population (densities
+ *   and supports are left unchanged): each column variant with the handle's
+ *   row variant, then each row variant with the chosen column variant. It
+ *   keeps the fastest pair (per pass, the lowest-numbered variant within
+ *   1.5 % of the fastest) for this handle and makes it the default of handles
+ *   of that size created later in the process; col_ms / row_ms (NULL or one
+ *   entry per variant) receive the mean iteration durations in ms.
  * spr_set_variants / spr_get_variants: one handle's pair.
  * spr_default_variants / spr_set_default_variants: the pair new handles of
  *   size n start with (initially the table compiled from the last autotuning

@@ -2,10 +2,14 @@
 
 For every supported size from 256 up, a population of synthetic patterns of
 about 2^24 pixels in total (64 individuals at 512^2) is made resident and
-spr_autotune times every column and row variant on it (`--reps` launches each,
-`--rounds` rounds; a variant's time is its median over the rounds). The fastest
-variant of each pass (the lowest-numbered one within 1.5 % of the fastest) per size is written as the compiled default table (spr_tuned.hpp) and the
-timings as JSON; rebuild the library afterwards.
+spr_autotune times every column and row variant on it in whole iterations (a
+column pass then a row pass, `--reps` iterations each, `--rounds` rounds; a
+variant's time is its median over the rounds): the column variants with the
+handle's row variant, the row variants with the chosen column variant. The
+fastest variant of each pass (the lowest-numbered one within 1.5 % of the
+fastest) per size is written as the compiled default table (spr_tuned.hpp) and
+the iteration times as JSON (`column_ms`, `row_ms`: ms per iteration with each
+variant); rebuild the library afterwards.
 
     python -m s_spi_spring_amd.autotune --header OUT.hpp --json OUT.json
 """

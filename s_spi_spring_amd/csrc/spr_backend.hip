@@ -911,8 +911,15 @@ VariantPair default_variants(int n) {
 }
 
 // Mean duration (ms) of `reps` launches of one pass in the handle's current variant.
-int time_pass(spr_ctx* c, bool column, const RowArgs& ra, int reps, hipEvent_t e0, hipEvent_t e1, float* ms) {
-    auto once = [&]() { return column ? col_n(c, COL_PROJ, c->work, c->P, c->stream) : row_n(c, ROW_RAAR, ra, c->P, c->stream); };
+// Average time of one iteration (column pass, then RAAR row pass) with the
+// handle's current variants. Timing whole iterations, not one pass repeated,
+// keeps the caches as the iteration leaves them: a pass that reads a table from
+// global memory finds it evicted by the other pass, as it does in a block.
+int time_iteration(spr_ctx* c, const RowArgs& ra, int reps, hipEvent_t e0, hipEvent_t e1, float* ms) {
+    auto once = [&]() {
+        SPR_TRY(col_n(c, COL_PROJ, c->work, c->P, c->stream));
+        return row_n(c, ROW_RAAR, ra, c->P, c->stream);
+    };
     SPR_TRY(once());   // warm-up (code load, caches)
     SPR_HIP(c, hipEventRecord(e0, c->stream));
     for (int i = 0; i < reps; ++i) SPR_TRY(once());
@@ -925,9 +932,11 @@ int time_pass(spr_ctx* c, bool column, const RowArgs& ra, int reps, hipEvent_t e
 }
 
 // Times every column and row variant of the handle's size on its resident
-// population and keeps the fastest. The densities and supports are not
-// changed: the row variants update a copy of the densities in work2 and the
-// column variants transform `work`, which every operation rewrites first.
+// population and keeps the fastest: first the column variants, each in whole
+// iterations with the handle's row variant, then the row variants with the
+// chosen column variant. The densities and supports are not changed: the row
+// passes update a copy of the densities in work2 and the column passes
+// transform `work`, which every operation rewrites first.
 int autotune(spr_ctx* c, int reps, float* col_ms, float* row_ms) {
     if (c->P <= 0) return fail(c, SPR_ERR_ARG, "autotune needs resident individuals");
     if (reps < 1) return fail(c, SPR_ERR_ARG, "reps must be >= 1");
@@ -951,19 +960,6 @@ int autotune(spr_ctx* c, int reps, float* col_ms, float* row_ms) {
         (void)hipEventDestroy(e0);
         return fail(c, SPR_ERR_HIP, "hipEventCreate failed");
     }
-    float tc[MAX_VARIANTS] = {}, tr[MAX_VARIANTS] = {};
-    int rc = SPR_OK;
-    for (int v = 0; v < nc && rc == SPR_OK; ++v) {
-        c->colvar = v;
-        rc = time_pass(c, true, ra, reps, e0, e1, &tc[v]);
-        if (col_ms) col_ms[v] = tc[v];
-    }
-    c->colvar = keep.col;
-    for (int v = 0; v < nr && rc == SPR_OK; ++v) {
-        c->rowvar = v;
-        rc = time_pass(c, false, ra, reps, e0, e1, &tr[v]);
-        if (row_ms) row_ms[v] = tr[v];
-    }
     // The lowest-numbered variant within TIE of the fastest: near-ties are timing
     // noise, and preferring the lower index keeps the choice stable between runs.
     constexpr float TIE = 1.015f;
@@ -974,11 +970,23 @@ int autotune(spr_ctx* c, int reps, float* col_ms, float* row_ms) {
             if (t[v] <= m * TIE) return v;
         return 0;
     };
+    float tc[MAX_VARIANTS] = {}, tr[MAX_VARIANTS] = {};
+    int rc = SPR_OK;
     VariantPair best;
-    if (rc == SPR_OK) {
-        best.col = pick(tc, nc);
-        best.row = pick(tr, nr);
+    c->rowvar = keep.row;
+    for (int v = 0; v < nc && rc == SPR_OK; ++v) {
+        c->colvar = v;
+        rc = time_iteration(c, ra, reps, e0, e1, &tc[v]);
+        if (col_ms) col_ms[v] = tc[v];
     }
+    if (rc == SPR_OK) best.col = pick(tc, nc);
+    c->colvar = best.col;
+    for (int v = 0; v < nr && rc == SPR_OK; ++v) {
+        c->rowvar = v;
+        rc = time_iteration(c, ra, reps, e0, e1, &tr[v]);
+        if (row_ms) row_ms[v] = tr[v];
+    }
+    if (rc == SPR_OK) best.row = pick(tr, nr);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc != SPR_OK) {

@@ -19,6 +19,8 @@
 
 #include <cstdint>
 
+#include <type_traits>
+
 #include "spr_fft.hpp"
 
 namespace spr {
@@ -33,9 +35,11 @@ constexpr int pow2_floor(int x) { return x >= 2 ? 2 * pow2_floor(x / 2) : 1; }
 // row-pass workgroup, row-pass prefetch mode RPF (-1: SPR_ROW_PREFETCH), PAD
 // float2 of padding per row of the column tile, and TWG = 1 to read the column
 // pass's twiddle table from global memory (through the caches) instead of
-// building it in LDS.
+// building it in LDS. CSK > 0 lays the column tile out unpadded with one spare
+// tile row after every 2^CSK rows (row i of the tile at (i + i/2^CSK) * C; PAD
+// is then unused).
 template <int N, class PL_ = RadixPlan<N>, int CCOLS = 0, int RTHREADS = 256, int RPF = -1, int PAD = 1, int TWG = 0,
-          int RSH = 3>
+          int RSH = 3, int CSK = 0>
 struct GeoT {
     using PL = PL_;
     static constexpr bool TW_GLOBAL = TWG != 0;
@@ -63,6 +67,7 @@ struct GeoT {
     static constexpr bool COL_WAVE = SPR_COL_WAVE && T <= 64;
     static constexpr int COL_THREADS = C * T;
     static constexpr int CS = C + PAD;                    // padded LDS row length of a column tile
+    static constexpr int COL_SKEW = CSK;
     // Column-major LDS tile of the wavefront-owned column pass (T <= 64): column
     // c, row i at c*CST + i + i/8. Within a column this is the row pass's padded
     // layout; the column stride CST = RPW (mod 2*RPW) float2, RPW = 64/C rows
@@ -71,7 +76,7 @@ struct GeoT {
     static constexpr int RPW = (C >= 64) ? 1 : 64 / C;
     static constexpr int CST = ((N + N / 8) / (2 * RPW) + 1) * (2 * RPW) + RPW;
     // Two column workgroups per CU when their LDS fits (160 KB): waves per SIMD to target.
-    static constexpr int COL_TILE = COL_WAVE ? C * CST : N * CS;   // float2 of the column tile
+    static constexpr int COL_TILE = COL_WAVE ? C * CST : CSK ? (N + (N >> CSK)) * C : N * CS;   // float2 of the column tile
     static constexpr int COL_LDS = (COL_TILE + (TWG ? 0 : TwTableP<PL>::SIZE)) * 8 + 512;
 #ifndef SPR_COL_WG_PER_CU
 #define SPR_COL_WG_PER_CU 8
@@ -120,7 +125,7 @@ SPR_COL_VARIANT(768, 2, RadixPlan<768>, 4)
 SPR_COL_VARIANT(1024, 1, Plan<1024, 16, 16, 8, 8>, 16)
 SPR_COL_VARIANT(1024, 2, Plan<1024, 16, 16, 8, 8>, 8)
 SPR_COL_VARIANT(1024, 3, RadixPlan<1024>, 4)
-SPR_COL_VARIANT(1024, 4, Plan<1024, 16, 16, 8, 8>, 4)
+SPR_COL_VARIANT(1024, 4, Plan<1024, 16, 16, 8, 8>, 8, 256, -1, 0, 0, 3, 4)
 // (plan, columns, -, -, padding, global twiddles): tiles under 80 KB, two workgroups per CU
 SPR_COL_VARIANT(384, 4, RadixPlan<384>, 16, 256, -1, 1, 1)
 SPR_COL_VARIANT(512, 4, RadixPlan<512>, 16, 256, -1, 1, 1)
@@ -133,6 +138,13 @@ SPR_COL_VARIANT(1024, 6, Plan<1024, 16, 16, 8, 8>, 8, 256, -1, 0, 0)
 SPR_COL_VARIANT(1024, 7, RadixPlan<1024>, 8, 256, -1, 1, 1)
 SPR_COL_VARIANT(2048, 1, RadixPlan<2048>, 4, 256, -1, 0, 1)
 SPR_COL_VARIANT(2048, 2, Plan<2048, 16, 16, 8, 16>, 8, 256, -1, 0, 1)
+// (plan, columns, -, -, -, -, -, spare tile row every 2^k rows): tiles without
+// the bank conflicts of the padded layouts. Measured on MI355X the conflicts
+// cost little (the pass is bound by HBM and VALU): 768^2 4 columns 73.7 vs
+// 76.7 us, 1024^2 radix 16 8 columns 68.7 vs 69.6 us, 512^2 8 columns 56.6 vs
+// 57.4 us; at 256^2, 384^2 and 2048^2 the skewed tiles were slower.
+SPR_COL_VARIANT(512, 6, RadixPlan<512>, 8, 256, -1, 0, 0, 3, 3)
+SPR_COL_VARIANT(768, 6, RadixPlan<768>, 4, 256, -1, 0, 0, 3, 2)
 // row variants: plan, -, threads per workgroup, prefetch mode
 SPR_ROW_VARIANT(256, 1, RadixPlan<256>, 0, 512)
 SPR_ROW_VARIANT(384, 1, RadixPlan<384>, 0, 512)
@@ -215,6 +227,24 @@ template <int SH> struct RowAccT {
     }
 };
 using RowAcc = RowAccT<3>;
+
+// Column c of a tile of C columns stored unpadded, with one spare tile row after
+// every 2^SH rows: element i at (i + (i >> SH)) * C. Two rows 2^SH apart then
+// fall C float2 apart modulo 2C, so the rows that the lanes of one LDS access
+// touch together spread over the banks.
+template <int C, int SH> struct ColAccSkew {
+    float2* base;
+    __device__ __forceinline__ float2 ld(int i) const { return base[(i + (i >> SH)) * C]; }
+    __device__ __forceinline__ void st(int i, float2 v) const { base[(i + (i >> SH)) * C] = v; }
+    template <int STEP> __device__ __forceinline__ float2 ldk(int i0, int k) const {
+        static_assert(STEP % (1 << SH) == 0, "stride");
+        return base[(i0 + (i0 >> SH) + k * (STEP + (STEP >> SH))) * C];
+    }
+    template <int STEP> __device__ __forceinline__ void stk(int i0, int k, float2 v) const {
+        static_assert(STEP % (1 << SH) == 0, "stride");
+        base[(i0 + (i0 >> SH) + k * (STEP + (STEP >> SH))) * C] = v;
+    }
+};
 
 template <int CS> struct ColAcc {
     float2* base;
@@ -498,7 +528,8 @@ __attribute__((amdgpu_waves_per_eu((ColVariant<N, CV>::G::COL_WAVES_PER_EU)))) v
         for (int r = 0; r < V; ++r) v[r] = mine.ldk<T>(t, r);
     } else {
         const float* __restrict__ amp = a.amp + amp_offset<N>(a.amp_of, p) + x0;
-        const ColAcc<CS> acc{lds + c};
+        using Acc = std::conditional_t<(G::COL_SKEW > 0), ColAccSkew<C, (G::COL_SKEW > 0 ? G::COL_SKEW : 1)>, ColAcc<CS>>;
+        const Acc acc{lds + c};
         float am[V];
         if constexpr (MODE == COL_PROJ && SPR_COL_AMP_PREFETCH) {
 #pragma unroll
