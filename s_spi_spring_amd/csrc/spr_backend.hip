@@ -204,6 +204,15 @@ template <int N> int launch_row(spr_ctx* c, int mode, const RowArgs& a, int cnt,
         case 4:
             if constexpr (RowVariant<N, 4>::valid) return launch_row_v<N, 4>(c, mode, a, cnt, s);
             break;
+        case 5:
+            if constexpr (RowVariant<N, 5>::valid) return launch_row_v<N, 5>(c, mode, a, cnt, s);
+            break;
+        case 6:
+            if constexpr (RowVariant<N, 6>::valid) return launch_row_v<N, 6>(c, mode, a, cnt, s);
+            break;
+        case 7:
+            if constexpr (RowVariant<N, 7>::valid) return launch_row_v<N, 7>(c, mode, a, cnt, s);
+            break;
     }
     return fail(c, SPR_ERR_ARG, "bad row variant");
 }

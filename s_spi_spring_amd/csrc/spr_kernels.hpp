@@ -158,6 +158,14 @@ SPR_ROW_VARIANT(2048, 1, Plan<2048, 16, 16, 8, 16>)
 // (plan, -, threads, prefetch, -, -, padding period 2^4)
 SPR_ROW_VARIANT(1024, 4, Plan<1024, 16, 16, 8, 8>, 0, 256, -1, 1, 0, 4)
 SPR_ROW_VARIANT(2048, 2, Plan<2048, 16, 16, 8, 16>, 0, 256, -1, 1, 0, 4)
+// default plans with the padding period that a model of the LDS banks finds
+// fewest conflicts for (2^4, or 2^5 for the radix-12 plans)
+SPR_ROW_VARIANT(256, 2, RadixPlan<256>, 0, 256, -1, 1, 0, 4)
+SPR_ROW_VARIANT(384, 2, RadixPlan<384>, 0, 256, -1, 1, 0, 5)
+SPR_ROW_VARIANT(512, 2, RadixPlan<512>, 0, 256, -1, 1, 0, 4)
+SPR_ROW_VARIANT(768, 3, RadixPlan<768>, 0, 256, -1, 1, 0, 5)
+SPR_ROW_VARIANT(1024, 5, RadixPlan<1024>, 0, 256, -1, 1, 0, 4)
+SPR_ROW_VARIANT(2048, 3, RadixPlan<2048>, 0, 256, -1, 1, 0, 4)
 #undef SPR_COL_VARIANT
 #undef SPR_ROW_VARIANT
 
@@ -179,6 +187,9 @@ template <int N> constexpr int row_variants() {
     if constexpr (RowVariant<N, 2>::valid) k = 3;
     if constexpr (RowVariant<N, 3>::valid) k = 4;
     if constexpr (RowVariant<N, 4>::valid) k = 5;
+    if constexpr (RowVariant<N, 5>::valid) k = 6;
+    if constexpr (RowVariant<N, 6>::valid) k = 7;
+    if constexpr (RowVariant<N, 7>::valid) k = 8;
     return k;
 }
 

@@ -9,12 +9,12 @@ struct TunedVariants {
     int n, col, row;
 };
 constexpr TunedVariants kTuned[] = {
-    {256, 1, 0},
-    {384, 1, 0},
+    {256, 1, 2},
+    {384, 1, 2},
     {512, 6, 0},
-    {768, 6, 0},
-    {1024, 2, 0},
-    {2048, 2, 1},
+    {768, 6, 3},
+    {1024, 2, 1},
+    {2048, 2, 0},
 };
 
 }  // namespace spr
