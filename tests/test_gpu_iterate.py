@@ -127,3 +127,34 @@ def test_chunked_blocks_are_bit_identical(hip, alg, beta):
         be.close()
     for got in out[1:]:
         assert np.array_equal(got[0], out[0][0]) and np.array_equal(got[1], out[0][1])
+
+
+@pytest.mark.parametrize("n,P,fused", [(128, 4, 1), (128, 4, 0), (256, 3, 0), (512, 2, 0)])
+@pytest.mark.parametrize("alg,beta", [("HIO", 0.9), ("RAAR", 0.87), ("ER", 0.0)])
+def test_bounded_and_free_pixels(hip, orc, n, P, fused, alg, beta):
+    """P_M's three cases in one field: measured pixels (a >= 0), free pixels
+    (-1.5 <= a < 0: F kept) and upper bounds (a < -1.5: |F| clamped to -a-2
+    when above it), on the on-chip (fused=1) and the two-pass kernels."""
+    print(f"bounded pixels: n={n} P={P} fused={fused} {alg}", flush=True)
+    amp, rho, supp = case(n, P, seed=41)
+    rng = np.random.default_rng(41)
+    amp = amp.copy()
+    sel = rng.random(amp.shape) < 0.08
+    bound = np.abs(amp[sel]) * rng.uniform(0.3, 1.5, sel.sum()).astype(np.float32) + np.float32(0.5)
+    amp[sel] = -np.float32(2.0) - bound
+    free = (rng.random(amp.shape) < 0.04) & ~sel
+    amp[free] = -1.0
+    assert (amp < -1.5).sum() > 0 and ((amp < 0) & (amp >= -1.5)).sum() > 0
+    be = hip.Backend(amp, capacity=P)
+    if n <= 128:
+        be.set_fused(fused)
+    be.set_population(rho, supp)
+    assert be.fused()[1] == bool(fused)
+    be.iterate(alg, 1, beta, 0.0)
+    got = be.get_population(support=False)
+    want64 = orc.iterate(alg, rho.astype(np.complex128), supp, amp.astype(np.float64), 1, beta, 0.0)
+    want32 = orc.iterate(alg, rho, supp, amp, 1, beta, 0.0)
+    e32, e64 = rel_l2(got, want32), rel_l2(got, want64)
+    print(f"  rel-L2 vs fp32 oracle {e32:.2e}, vs fp64 {e64:.2e}")
+    assert e32 < TOL and e64 < TOL
+    be.close()
