@@ -107,3 +107,44 @@ def test_low_count_gradient_1024(hip):
             fd = (loss64(r1) - loss64(r2)) / (2 * h)
             got = g[y, xx].real if d == 1.0 else g[y, xx].imag
             assert abs(fd - got) <= 1e-3 * scale, (p, d, fd, got)
+
+
+def test_schedule_precision_1024(hip, orc):
+    """A short schedule at 1024^2 (1e10 photons), two individuals: RAAR 25, SW,
+    RAAR 25, SW, HIO 25, ER 25, by the FP32 GPU path, the FP32 oracle and the
+    FP64 oracle from the same start. HIO and RAAR amplify rounding, so the bars
+    are relative: the GPU's final densities and Fourier errors are as close to
+    the FP64 run as the FP32 oracle's are (within 4x), and the supports after
+    shrink-wrap differ from the FP64 run's in few pixels."""
+    n, P = 1024, 2
+    amp, rho, supp = octa_case(n, P, seed=35)
+    seq = [("RAAR", [25, 0.87]), ("SW", [3.0, 0.03]), ("RAAR", [25, 0.87]), ("SW", [2.0, 0.03]),
+           ("HIO", [25, 0.95]), ("ER", [25])]
+    be = hip.Backend(amp, capacity=P)
+    be.set_population(rho, supp)
+    be.run(seq)
+    got, gs = be.get_population()
+    be.close()
+
+    def oracle_run(r, s, a):
+        for name, p in seq:
+            if name == "SW":
+                s = np.stack([orc.sw(r[i], s[i], a, p[0], p[1]) for i in range(P)])
+            else:
+                r = orc.iterate(name, r, s, a, int(p[0]), p[1] if len(p) > 1 else 0.0, 0.0)
+        return r, s
+
+    r64, s64 = oracle_run(rho.astype(np.complex128), supp, amp.astype(np.float64))
+    r32, s32 = oracle_run(rho, supp, amp)
+    e_gpu, e_32 = rel_l2(got, r64), rel_l2(r32, r64)
+    ef = lambda r, s: np.array([orc.get_error(r[i].astype(np.complex128), s[i], amp.astype(np.float64))[0]
+                                for i in range(P)])
+    f64, fg, f32 = ef(r64, s64), ef(got, gs), ef(r32, s32)
+    d_gpu, d_32 = np.abs(fg - f64).max(), np.abs(f32 - f64).max()
+    flips = int((gs != (s64 != 0)).sum())
+    print(f"1024^2 schedule: densities rel-L2 vs FP64: GPU {e_gpu:.2e}, FP32 oracle {e_32:.2e}; "
+          f"E_F FP64 {np.round(f64, 5).tolist()} GPU {np.round(fg, 5).tolist()} FP32 {np.round(f32, 5).tolist()}; "
+          f"support pixels differing from FP64: {flips} of {int((s64 != 0).sum())}")
+    assert e_gpu < 4 * max(e_32, 1e-6)
+    assert d_gpu <= 4 * max(d_32, 1e-4 * f64.max())
+    assert flips <= 1e-3 * (s64 != 0).sum() + 8
