@@ -239,6 +239,31 @@ def time_to_rf_study(seeds, generations, target, log, threshold=None, groups=())
     print(json.dumps(out), flush=True)
 
 
+def differentiation_study(seeds, generations, log):
+    """The low-count differentiation workload (1024^2 truncated octahedron,
+    1e8 photons, population 32, RAAR blocks alternating with 20 Adam steps on
+    the Poisson likelihood): best Fourier error and time per generation; one
+    JSON line."""
+    from s_spi_spring_amd import workloads
+    runs = []
+    for seed in range(seeds):
+        r = workloads.differentiation_run(seed, generations=generations)
+        runs.append(r)
+        log(f"seed {seed}: E_F by generation {r['history']} at {r['times']} s; real-space {r['real_space_error']:.3f} "
+            f"(noise floor {r['noise_floor']:.4f})")
+    per_gen = [np.diff(r["times"]).tolist() for r in runs]
+    out = {"metric": "s per memetic generation, low-count differentiation workload", "unit": "s",
+           "higher_is_better": False, "seeds": seeds, "generations": generations,
+           "median_s_per_generation": float(np.median([t for ts in per_gen for t in ts])),
+           "config": {"workload": "truncated-octahedron 1024^2 pattern, 1e8 photons, population 32, per generation "
+                                  "8 x [RAAR 25, 20 support-projected Adam steps on the Poisson likelihood (step 1e-2 "
+                                  "of the RMS density), shrink-wrap] + ER 50"},
+           "iterations_per_generation": runs[0]["iterations_per_generation"],
+           "adam_steps_per_generation": runs[0]["adam_steps_per_generation"],
+           "per_seed": runs}
+    print(json.dumps(out), flush=True)
+
+
 def batch_bench(args, log):
     """Mixed-size batch: `args.batch` independent polygon patterns with sizes
     drawn from {256, 384, 512, 768, 1024} and `args.batch_pop` individuals each,
@@ -326,6 +351,8 @@ def main():
     ap.add_argument("--generations", type=int, default=10)
     ap.add_argument("--sw-threshold", type=float, default=None,
                     help="shrink-wrap threshold of the time-to-R_F schedules (default 0.03)")
+    ap.add_argument("--differentiation", type=int, default=0, metavar="SEEDS",
+                    help="run the low-count 1024^2 differentiation workload on SEEDS patterns (--generations)")
     ap.add_argument("--batch", type=int, default=0, metavar="PATTERNS",
                     help="mixed-size batch workload of PATTERNS patterns (256 in the batching configuration)")
     ap.add_argument("--batch-pop", type=int, default=32)
@@ -357,6 +384,9 @@ def main():
 
     if args.batch:
         batch_bench(args, log)
+        return
+    if args.differentiation:
+        differentiation_study(args.differentiation, args.generations, log)
         return
 
     from s_spi_spring_amd import hipcore, synth

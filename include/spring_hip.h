@@ -229,11 +229,13 @@ int spr_loss_grad(spr_ctx* ctx, int P, const float* rho, float* grad, double* lo
  * doubles): the loss after the last step. */
 int spr_refine(spr_ctx* ctx, int steps, float lr, int kind, float eps, double* loss);
 
-/* Support-projected refinement (the GRAD operation of spr_run): as spr_refine,
- * but only pixels inside each individual's support move (the gradient is
- * masked by the support), and the step size of individual p is
- * lr_rel * sqrt(sum_S |rho_p|^2 / |S_p|), the RMS density on its support when
- * the call starts. No counterpart in the reference (new operator). */
+/* Support-projected refinement (the GRAD operation of spr_run): projected
+ * Adam on the loss above over densities that vanish outside the support. Each
+ * density is first set to 0 outside its individual's support, then refined as
+ * by spr_refine with the gradient masked by the support (pixels outside stay
+ * 0); the step size of individual p is lr_rel * sqrt(sum_S |rho_p|^2 / |S_p|),
+ * the RMS density on its support when the call starts. No counterpart in the
+ * reference (new operator). */
 int spr_refine_support(spr_ctx* ctx, int steps, float lr_rel, int kind, float eps, double* loss);
 
 /* Device addresses of the resident population: rho (P*n*n float2) and the

@@ -1862,6 +1862,14 @@ int spr_refine_support(spr_ctx* c, int steps, float lr_rel, int kind, float eps,
     SPR_HIP(c, hipSetDevice(c->device));
     SPR_TRY(ensure(c, c->xphase, (size_t)P * sizeof(float)));
     float* lrp = (float*)c->xphase.p;
+    // Start from the projection onto the supports: after an ER/HIO/RAAR block
+    // the densities are IFT(P_M(FT rho)), whose Fourier magnitudes already
+    // minimise the loss, so descending from them without the support
+    // constraint has nothing to gain.
+    const size_t total = (size_t)P * np;
+    hipLaunchKernelGGL(k_zero_outside, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->stream, c->rho,
+                       (const uint8_t*)c->supp, total);
+    SPR_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(k_support_scale, dim3(P), dim3(256), 0, c->stream, (const float2*)c->rho,
                        (const uint8_t*)c->supp, np, lr_rel, lrp);
     SPR_HIP(c, hipGetLastError());

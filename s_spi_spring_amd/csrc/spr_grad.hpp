@@ -79,6 +79,13 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_loss(LossArgs a) {
     for (int q = 0; q < V; ++q) f[off + q * T * N] = v[q];
 }
 
+// Projection onto the supports: rho <- 0 outside each individual's support (the
+// start of the support-projected refinement). total = P * N^2.
+__global__ __launch_bounds__(256) void k_zero_outside(float2* rho, const uint8_t* supp, size_t total) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < total && supp[i] == 0) rho[i] = make_float2(0.f, 0.f);
+}
+
 // Per-individual step size of the support-projected refinement:
 // lr_p = lr_rel * sqrt(sum_{S} |rho|^2 / |S|) (0 for an empty support), summed
 // in double in a fixed order (one workgroup per individual).

@@ -64,11 +64,12 @@ def refine(rho, backend, steps=20, lr=1e-2, kind="poisson", eps=1e-3):
 
 def refine_support(rho, support, backend, steps=20, lr_rel=1e-2, kind="poisson", eps=1e-3):
     """Support-projected Adam refinement (the GRAD operation of the memetic loop)
-    written with torch.optim.Adam: one parameter group per field with step
+    written with torch.optim.Adam: the densities set to 0 outside their
+    supports, then one parameter group per field with step
     lr_rel * RMS(rho on its support), and the gradient masked by the support.
     rho: complex64 [P, N, N] on the GPU; support: 0/1 [P, N, N]."""
-    x = rho.detach().clone()
-    S = torch.as_tensor(support, device=x.device).to(torch.float32)
+    S = torch.as_tensor(support, device=rho.device).to(torch.float32)
+    x = rho.detach().clone() * S
     params, groups = [], []
     for p in range(x.shape[0]):
         sp = S[p] > 0

@@ -72,6 +72,18 @@ def iterations(seq):
     return sum(int(p[0]) for name, p in seq if name in ("ER", "HIO", "RAAR"))
 
 
+def differentiation_1024(generation=0, area=None, threshold=None, blocks=8, steps=20, lr=1e-2):
+    """The low-count differentiation workload's generation: `blocks` x [RAAR 25,
+    `steps` steps of support-projected Adam on the masked Poisson likelihood
+    (step lr x the RMS density on the support), shrink-wrap], then ER 25 x 2."""
+    th = SW_THRESHOLD_512 if threshold is None else threshold
+    body = [[("RAAR", [25, 0.87]), ("GRAD", [steps, lr, 0])] for _ in range(blocks)] + [[("ER", [25])]] * 2
+    seq = []
+    for k, b in enumerate(body):
+        seq += b + [("SW", [_sigma(k, len(body)), th])]
+    return seq
+
+
 SCHEDULES = {"plain": plain_512, "grad": grad_512}
 
 
@@ -124,3 +136,26 @@ def time_to_rf(seed, form="plain", target=0.05, generations=10, popsize=64, n=51
             "history": [round(x["best_fourier_error"], 5) for x in m.history],
             "noise_floor": noise_floor(I, truth),
             "iterations": iterations(sched()) * (h["generation"] + 1)}
+
+
+def differentiation_run(seed, generations=2, popsize=32, n=1024, photons=1e8):
+    """The memetic loop on a low-count truncated-octahedron pattern with the
+    differentiation_1024 schedule; returns the best Fourier error and the
+    wall time after every generation and the real-space error of the best
+    individual at the end."""
+    from . import Pattern, Settings, metrics, synth
+    from .mpr import MPR
+    I, truth = synth.octahedron_pattern(seed=seed, n=n, photons=photons)
+    s = Settings()
+    s.set("global", "popsize", popsize).set("global", "generations", generations)
+    s.set("init", "supportsize", support_size(truth)).set("IA", "reality", 0.0)
+    s.set_advanced("seed", 2000 + seed)
+    m = MPR(Pattern.from_intensity(I), s)
+    res = m.run(log=False, schedule=lambda g, area: differentiation_1024(g, area))
+    seq = differentiation_1024()
+    return {"seed": seed, "history": [round(h["best_fourier_error"], 5) for h in m.history],
+            "times": [round(h["time"], 3) for h in m.history],
+            "real_space_error": metrics.real_space_error(res.get_density_best(crop=False), truth),
+            "noise_floor": noise_floor(I, truth),
+            "iterations_per_generation": iterations(seq),
+            "adam_steps_per_generation": sum(int(p[0]) for name, p in seq if name == "GRAD")}
