@@ -142,8 +142,8 @@ def test_low_count_refinement_lowers_the_poisson_loss(hip):
 def test_differentiation_workload_runs(hip):
     """Two generations of the differentiation workload's memetic loop (RAAR
     blocks alternating with 20 Adam steps) at 1024^2 with 1e8 photons, with a
-    small population: every generation's best Fourier error is finite and below
-    that of the start."""
+    small population: every generation's best Fourier error is finite and well
+    below 1 (0.20 measured; the noise floor of this pattern is 0.23)."""
     from s_spi_spring_amd import workloads
     r = workloads.differentiation_run(0, generations=1, popsize=4)
     print(r["history"], r["times"], r["real_space_error"], r["noise_floor"])
