@@ -31,3 +31,17 @@ def test_schedules():
     assert workloads.iterations(p) == 550 and [x[0] for x in p].count("SW") == 22
     g = workloads.grad_512()
     assert workloads.iterations(g) == 550 and [x[0] for x in g].count("GRAD") == 16
+
+
+def test_differentiation_schedule():
+    """The low-count differentiation workload's generation: 8 x [RAAR 25, GRAD 20
+    (step 1e-2, Poisson), SW] + 2 x [ER 25, SW]."""
+    from s_spi_spring_amd import workloads
+    seq = workloads.differentiation_1024()
+    names = [name for name, _ in seq]
+    assert workloads.iterations(seq) == 250
+    assert names.count("GRAD") == 8 and names.count("SW") == 10
+    assert all(p == [20, 1e-2, 0] for name, p in seq if name == "GRAD")
+    assert names[-4:] == ["ER", "SW", "ER", "SW"]
+    sig = [p[0] for name, p in seq if name == "SW"]
+    assert sig[0] == 3.0 and sig[-1] == 1.5 and all(a >= b for a, b in zip(sig, sig[1:]))
