@@ -218,7 +218,8 @@ int spr_crossover(spr_ctx* ctx, int nchild, const spr_cross_desc* desc, const in
  *   kind 1 (amplitude)    L = sum (|F| - A)^2
  * grad = dL/dRe(rho) + i dL/dIm(rho) = 2 FT^H[f'(u) F] (PyTorch's convention for
  * complex tensors). rho and grad are device arrays of P fields (P <= capacity);
- * loss (optional, host, P doubles, FP64 sums). Runs on `stream` (a hipStream_t;
+ * grad may be NULL for the losses alone (the adjoint pass is skipped); loss
+ * (optional, host, P doubles, FP64 sums). Runs on `stream` (a hipStream_t;
  * NULL: the handle's stream) and returns after the work is queued unless loss
  * is requested. */
 int spr_loss_grad(spr_ctx* ctx, int P, const float* rho, float* grad, double* loss, int kind, float eps,

@@ -50,6 +50,17 @@ void Solver::check(int rc) const {
     if (rc != SPR_OK) throw std::runtime_error(std::string("spring_hip: ") + spr_last_error(ctx));
 }
 
+// An operation list that failed on the device may have changed some resident
+// individuals before it stopped: slots [0, P) no longer match their host
+// objects, so they are forgotten before the error is raised.
+void Solver::check_run(int rc, int P) {
+    if (rc != SPR_OK) {
+        const std::string msg = std::string("spring_hip: ") + spr_last_error(ctx);
+        forget(P);
+        throw std::runtime_error(msg);
+    }
+}
+
 // Solver::Solver (src/solver.cpp:9-52). One device per Solver: the reference
 // distributes its OpenMP contexts round-robin over GPU_ids; this backend runs
 // the whole population on the first listed device.
@@ -64,6 +75,10 @@ Solver::Solver(const FLOAT* pattern_in, std::array<std::size_t, 2> dims_in, std:
         throw std::invalid_argument(
             "Solver: no device selected (gpus=[]). There is no CPU backend; pass gpus=[device ids].");
     device = GPU_ids[0];
+    if (GPU_ids.size() > 1)
+        std::cerr << "WARNING: Solver runs its whole population on device " << device << " only; the other "
+                  << GPU_ids.size() - 1 << " listed device(s) are not used. Run one Solver per device to use more."
+                  << std::endl;
     set_threads(nthreads_);
     pattern.assign(pattern_in, pattern_in + size);
     if (center[0] < 0) center[0] = (int)dims[0] / 2;
@@ -279,7 +294,7 @@ void Solver::run_op(const RecVec& rec, const spr_op& op) {
     const bool support_only = op.kind == SPR_OP_SW || op.kind == SPR_OP_SWA;
     for_groups(rec, [&](const RecVec& g) {
         place(g);
-        check(spr_run(ctx, &op, 1, 1, reality));
+        check_run(spr_run(ctx, &op, 1, 1, reality), (int)g.size());
         pull(g, !support_only, true);
     });
 }
@@ -308,28 +323,35 @@ RecVec Solver::SWA(RecVec rec, FLOAT sigma, int pixels) {
 
 // Solver::get_algorithm + set_sequence (src/solver.cpp:84-170): names with the
 // expected parameter count become operations; anything else is reported and
-// skipped. Counts and sizes are the float parameters truncated to int.
+// skipped. Counts and sizes are the float parameters truncated to int; a
+// negative iteration count runs no iteration (the reference's loop bound), only
+// the trailing projection.
 void Solver::set_sequence(const std::vector<SeqItem>& sequence) {
-    ops.clear();
+    std::vector<spr_op> next;
+    auto iters = [](FLOAT v) { return std::max(0, (int)v); };
     for (const SeqItem& it : sequence) {
         const std::string& name = std::get<0>(it);
         const std::vector<FLOAT>& p = std::get<1>(it);
-        if (name == "ER" && p.size() == 1) ops.push_back(spr_op{SPR_OP_ER, (int)p[0], 0, 0, 0, 0});
-        else if (name == "HIO" && p.size() == 2) ops.push_back(spr_op{SPR_OP_HIO, (int)p[0], p[1], 0, 0, 0});
-        else if (name == "RAAR" && p.size() == 2) ops.push_back(spr_op{SPR_OP_RAAR, (int)p[0], p[1], 0, 0, 0});
-        else if (name == "SW" && p.size() == 2) ops.push_back(spr_op{SPR_OP_SW, 0, 0, p[0], p[1], 0});
-        else if (name == "SWA" && p.size() == 2) ops.push_back(spr_op{SPR_OP_SWA, 0, 0, p[0], 0, (int)p[1]});
+        if (name == "ER" && p.size() == 1) next.push_back(spr_op{SPR_OP_ER, iters(p[0]), 0, 0, 0, 0});
+        else if (name == "HIO" && p.size() == 2) next.push_back(spr_op{SPR_OP_HIO, iters(p[0]), p[1], 0, 0, 0});
+        else if (name == "RAAR" && p.size() == 2) next.push_back(spr_op{SPR_OP_RAAR, iters(p[0]), p[1], 0, 0, 0});
+        else if (name == "SW" && p.size() == 2) next.push_back(spr_op{SPR_OP_SW, 0, 0, p[0], p[1], 0});
+        else if (name == "SWA" && p.size() == 2) next.push_back(spr_op{SPR_OP_SWA, 0, 0, p[0], 0, (int)p[1]});
         // Gradient refinement (no counterpart in the reference): GRAD [steps, lr(, loss kind)]
         // = Adam on the masked Poisson (kind 0) or amplitude (kind 1) Fourier loss.
-        else if (name == "GRAD" && (p.size() == 2 || p.size() == 3))
-            ops.push_back(spr_op{SPR_OP_GRAD, (int)p[0], p[1], 0, 1e-3f, p.size() == 3 ? (int)p[2] : 0});
-        else {
+        else if (name == "GRAD" && (p.size() == 2 || p.size() == 3)) {
+            const int steps = (int)p[0], kind = p.size() == 3 ? (int)p[2] : 0;
+            if (steps < 0 || (kind != 0 && kind != 1))
+                throw std::invalid_argument("GRAD needs steps >= 0 and loss kind 0 (Poisson) or 1 (amplitude)");
+            next.push_back(spr_op{SPR_OP_GRAD, steps, p[1], 0, 1e-3f, kind});
+        } else {
             std::cout << name << " [ ";
             for (FLOAT v : p) std::cout << v << " ";
             std::cout << "] ";
             std::cout << " NOT RECOGNIZED. Skipping." << std::flush;
         }
     }
+    ops = std::move(next);
 }
 
 // get_error + get_symmetry + the derived metrics of Solver::Sequence / Evaluate
@@ -359,7 +381,7 @@ RecVec Solver::Sequence(RecVec rec, int nrep) {
     if (rec.empty()) return rec;
     for_groups(rec, [&](const RecVec& g) {
         place(g);
-        check(spr_run(ctx, ops.data(), (int)ops.size(), nrep, reality));
+        check_run(spr_run(ctx, ops.data(), (int)ops.size(), nrep, reality), (int)g.size());
         evaluate_resident(g);
         pull(g, true, true);
     });

@@ -106,6 +106,7 @@ private:
     void run_op(const RecVec& rec, const spr_op& op);
     void evaluate_resident(const RecVec& rec);
     void check(int rc) const;
+    void check_run(int rc, int P);                           // check(rc), forgetting slots [0, P) on failure
     std::vector<int> sorted_by_error(const RecVec& rec) const;
     std::vector<COMPLEX> ft_batch(const std::vector<std::vector<COMPLEX>>& fields, bool inverse);
 

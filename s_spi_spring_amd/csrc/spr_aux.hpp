@@ -398,13 +398,14 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_aux(AuxColArgs a) {
 // ---------------------------------------------------------------------------
 // Small kernels
 // ---------------------------------------------------------------------------
-__global__ void k_fill_u32(unsigned* p, unsigned v, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+// p[0..n) <- v, grid-stride (any grid size covers any n).
+__global__ void k_fill_u32(unsigned* p, unsigned v, size_t n) {
+    const size_t step = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) p[i] = v;
 }
-__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, size_t n) {
+    const size_t step = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) p[i] = v;
 }
 
 // out[item*out_stride + out_off + k] = sum_j partial[(item*per_item + j)*K + k], fixed order.

@@ -445,15 +445,18 @@ CorrWin make_corr_win(int n, float minr, float maxr) {
     return w;
 }
 
-int fill_u32(spr_ctx* c, unsigned* p, unsigned v, int n) {
-    if (n <= 0) return SPR_OK;
-    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, c->stream, p, v, n);
+// Grid of a grid-stride fill: enough workgroups to cover the chip, at most one
+// per 256 elements.
+dim3 fill_grid(size_t n) { return dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)); }
+int fill_u32(spr_ctx* c, unsigned* p, unsigned v, size_t n) {
+    if (n == 0) return SPR_OK;
+    hipLaunchKernelGGL(k_fill_u32, fill_grid(n), dim3(256), 0, c->stream, p, v, n);
     SPR_HIP(c, hipGetLastError());
     return SPR_OK;
 }
-int fill_u64(spr_ctx* c, unsigned long long* p, unsigned long long v, int n) {
-    if (n <= 0) return SPR_OK;
-    hipLaunchKernelGGL(k_fill_u64, dim3((n + 255) / 256), dim3(256), 0, c->stream, p, v, n);
+int fill_u64(spr_ctx* c, unsigned long long* p, unsigned long long v, size_t n) {
+    if (n == 0) return SPR_OK;
+    hipLaunchKernelGGL(k_fill_u64, fill_grid(n), dim3(256), 0, c->stream, p, v, n);
     SPR_HIP(c, hipGetLastError());
     return SPR_OK;
 }
@@ -510,7 +513,7 @@ int queue_swa(spr_ctx* c, float sigma, int spixels) {
     const int bpi = 16;   // histogram blocks per individual
     const int los[3] = {21, 10, 0};
     for (int pass = 0; pass < 3; ++pass) {
-        SPR_TRY2(fill_u32(c, c->hist, 0u, c->P * RADIX_BINS));
+        SPR_TRY2(fill_u32(c, c->hist, 0u, (size_t)c->P * RADIX_BINS));
         const int lo = los[pass];
         if (pass < 2) {
             hipLaunchKernelGGL((k_radix_hist<11>), dim3(c->P * bpi), dim3(256), 0, c->stream, (const float*)c->work2,
@@ -561,7 +564,7 @@ int queue_symmetry(spr_ctx* c, float symmin, float symmax) {
     ca.ref = nullptr;
     ca.ref_index = nullptr;
     SPR_TRY2(col_aux(c, ACOL_CORR, ca, c->P));
-    SPR_TRY2(fill_u64(c, c->keys, 0ull, 2 * c->cap));
+    SPR_TRY2(fill_u64(c, c->keys, 0ull, 2 * (size_t)c->cap));
     SPR_TRY2(row_aux(c, AROW_CORR, a, c->P));
     return SPR_OK;
 }
@@ -783,7 +786,7 @@ int correlate_and_pick(spr_ctx* c, const float2* spec, const int* d_index, const
     ca.win = make_corr_win(c->n, minr, maxr);
     ca.unit_scale = unit_scale;
     SPR_TRY2(corr_spec(c, ca, items));
-    SPR_TRY2(fill_u64(c, c->keys, 0ull, 2 * c->cap));
+    SPR_TRY2(fill_u64(c, c->keys, 0ull, 2 * (size_t)c->cap));
     AuxRowArgs a = aux_row_args(c);
     SPR_TRY2(row_aux(c, AROW_CORR, a, items));
     hipLaunchKernelGGL(k_pick_shift, dim3((items + 63) / 64), dim3(64), 0, c->stream,
@@ -822,8 +825,8 @@ int queue_adam_steps(spr_ctx* c, int steps, float lr, const float* lrp, int kind
     const size_t n = (size_t)c->n * c->n * P;
     SPR_TRY2(ensure(c, c->adam_m, n * sizeof(float2)));
     SPR_TRY2(ensure(c, c->adam_v, n * sizeof(float2)));
-    SPR_TRY2(fill_u32(c, (unsigned*)c->adam_m.p, 0u, (int)(2 * n)));
-    SPR_TRY2(fill_u32(c, (unsigned*)c->adam_v.p, 0u, (int)(2 * n)));
+    SPR_TRY2(fill_u32(c, (unsigned*)c->adam_m.p, 0u, 2 * n));
+    SPR_TRY2(fill_u32(c, (unsigned*)c->adam_v.p, 0u, 2 * n));
     FieldBatch fwd{c->work, nullptr, c->tw, 1.f, c->rho};
     SPR_TRY2(fft_rows(c, +1, fwd, P));
     const float b1 = 0.9f, b2 = 0.999f;
@@ -850,17 +853,27 @@ struct StreamSwap {
     ~StreamSwap() { c->stream = saved; }
 };
 
-// grad <- 2 FT^H[f'(|F|^2) F], F = FT(rho), for P fields at device addresses;
+// The passes of the loss and its gradient for P fields at device addresses:
+// the row FFT of rho into `field`, the column loss pass on it (one loss
+// partial per workgroup into c->partial) and, unless grad is NULL, the
+// adjoint row pass grad <- 2 FT^H[f'(|F|^2) F]. grad may be `field` itself
+// (the adjoint pass then runs in place, row by row).
+int queue_loss_passes(spr_ctx* c, int P, const float2* rho, float2* field, float2* grad, int kind, float eps) {
+    FieldBatch fwd{field, nullptr, c->tw, 1.f, rho};
+    SPR_TRY2(fft_rows(c, +1, fwd, P));
+    LossArgs la{field, c->amp, c->amp_of, c->tw, c->partial, eps, kind};
+    SPR_TRY2(col_loss(c, la, P));
+    if (!grad) return SPR_OK;
+    FieldBatch adj{grad, nullptr, c->tw, 2.f, grad == field ? nullptr : field};
+    return fft_rows(c, -1, adj, P);
+}
+
+// grad <- 2 FT^H[f'(|F|^2) F], F = FT(rho) (grad NULL: the losses only);
 // per-field losses into sums[p*SUMS_PER_ITEM + 7].
 int queue_loss_grad(spr_ctx* c, int P, const float2* rho, float2* grad, int kind, float eps) {
     const int n = c->n;
-    FieldBatch fwd{c->work, nullptr, c->tw, 1.f, rho};
-    SPR_TRY2(fft_rows(c, +1, fwd, P));
-    LossArgs la{c->work, c->amp, c->amp_of, c->tw, c->partial, eps, kind};
-    SPR_TRY2(col_loss(c, la, P));
-    SPR_TRY2(reduce(c, n / cols_per_wg(n), 1, 7, P));
-    FieldBatch adj{grad, nullptr, c->tw, 2.f, c->work};
-    return fft_rows(c, -1, adj, P);
+    SPR_TRY2(queue_loss_passes(c, P, rho, c->work, grad, kind, eps));
+    return reduce(c, n / cols_per_wg(n), 1, 7, P);
 }
 
 // Loss of every resident density after a refinement (host, P doubles).
@@ -1491,14 +1504,26 @@ int spr_symmetry(spr_ctx* c, float symmin, float symmax, float* symmetry) {
     return harvest_timing(c);
 }
 
+// Every operation of a list is checked before any is queued, so that a bad
+// entry leaves the population untouched. NULL when the list is valid.
+static const char* op_list_error(const spr_op* ops, int nops) {
+    for (int i = 0; i < nops; ++i) {
+        const spr_op& o = ops[i];
+        if (o.kind < SPR_OP_ER || o.kind > SPR_OP_GRAD) return "unknown op kind";
+        if (o.kind <= SPR_OP_RAAR && o.iterations < 0) return "negative iteration count";
+        if (o.kind == SPR_OP_GRAD) {
+            if (o.iterations < 0) return "negative GRAD step count";
+            if (o.pixels != LOSS_POISSON && o.pixels != LOSS_AMPLITUDE) return "unknown GRAD loss kind";
+        }
+    }
+    return nullptr;
+}
+
 int spr_run(spr_ctx* c, const spr_op* ops, int nops, int nrep, float reality) {
     if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
     if (nops < 0 || nrep < 0 || (nops > 0 && !ops)) return fail(c, SPR_ERR_ARG, "bad op list");
-    for (int i = 0; i < nops; ++i) {
-        const spr_op& o = ops[i];
-        if (o.kind < SPR_OP_ER || o.kind > SPR_OP_GRAD) return fail(c, SPR_ERR_ARG, "unknown op kind");
-        if (o.kind <= SPR_OP_RAAR && o.iterations < 0) return fail(c, SPR_ERR_ARG, "negative iteration count");
-    }
+    const char* bad = op_list_error(ops, nops);
+    if (bad) return fail(c, SPR_ERR_ARG, bad);
     SPR_HIP(c, hipSetDevice(c->device));
     for (int r = 0; r < nrep; ++r) {
         for (int i = 0; i < nops; ++i) {
@@ -1824,7 +1849,7 @@ int spr_crossover(spr_ctx* c, int nchild, const spr_cross_desc* desc, const int3
 }
 
 int spr_loss_grad(spr_ctx* c, int P, const float* rho, float* grad, double* loss, int kind, float eps, void* stream) {
-    if (!c || !rho || !grad) return fail(c, SPR_ERR_ARG, "NULL argument");
+    if (!c || !rho) return fail(c, SPR_ERR_ARG, "NULL argument");
     if (P < 1 || P > c->cap) return fail(c, SPR_ERR_ARG, "field count exceeds the capacity");
     if (kind != LOSS_POISSON && kind != LOSS_AMPLITUDE) return fail(c, SPR_ERR_ARG, "unknown loss kind");
     SPR_HIP(c, hipSetDevice(c->device));
@@ -2290,11 +2315,8 @@ int spr_batch_get_population(spr_batch* b, int pattern, float* rho, int32_t* sup
 int spr_batch_run(spr_batch* b, const spr_op* ops, int nops, int nrep, float reality) {
     if (!b) return bfail(b, SPR_ERR_ARG, "NULL handle");
     if (nops < 0 || nrep < 0 || (nops > 0 && !ops)) return bfail(b, SPR_ERR_ARG, "bad op list");
-    for (int i = 0; i < nops; ++i) {
-        const spr_op& o = ops[i];
-        if (o.kind < SPR_OP_ER || o.kind > SPR_OP_GRAD) return bfail(b, SPR_ERR_ARG, "unknown op kind");
-        if (o.kind <= SPR_OP_RAAR && o.iterations < 0) return bfail(b, SPR_ERR_ARG, "negative iteration count");
-    }
+    const char* bad = op_list_error(ops, nops);
+    if (bad) return bfail(b, SPR_ERR_ARG, bad);
     SPR_BHIP(b, hipSetDevice(b->device));
     for (int r = 0; r < nrep; ++r) {
         for (int i = 0; i < nops; ++i) {

@@ -1,9 +1,11 @@
 """Differentiable masked Fourier-magnitude loss for PyTorch-ROCm.
 
-FourierLoss is a torch.autograd.Function whose forward and backward both run the
-hand-written HIP kernels behind spr_loss_grad (include/spring_hip.h): one call
-computes the per-field loss (FP64 sums) and its gradient through the adjoint
-FFT; backward scales the saved gradient by the incoming one. Losses (kind):
+FourierLoss is a torch.autograd.Function over the hand-written HIP kernels
+behind spr_loss_grad (include/spring_hip.h). When rho needs a gradient, one call
+in forward computes the per-field loss (FP64 sums) and its gradient through the
+adjoint FFT, and backward scales the saved gradient by the incoming one; when it
+does not (torch.no_grad, or a tensor without requires_grad), forward runs the
+loss passes only. Losses (kind):
 
   "poisson"    L = sum_{A>=0} |F|^2 - I log(|F|^2 + eps),  I = A^2 (counts)
   "amplitude"  L = sum_{A>=0} (|F| - A)^2
@@ -30,10 +32,13 @@ class FourierLoss(torch.autograd.Function):
         P = rho.shape[0]
         if tuple(rho.shape[1:]) != (backend.n, backend.n):
             raise ValueError("field size differs from the backend's pattern")
-        grad = torch.empty_like(rho)
         stream = torch.cuda.current_stream(rho.device).cuda_stream
-        loss = backend.loss_grad(P, rho.data_ptr(), grad.data_ptr(), KINDS[kind], eps, stream)
-        ctx.save_for_backward(grad)
+        if ctx.needs_input_grad[0]:
+            grad = torch.empty_like(rho)
+            loss = backend.loss_grad(P, rho.data_ptr(), grad.data_ptr(), KINDS[kind], eps, stream)
+            ctx.save_for_backward(grad)
+        else:
+            loss = backend.loss_grad(P, rho.data_ptr(), None, KINDS[kind], eps, stream)
         return torch.from_numpy(loss).to(rho.device)
 
     @staticmethod

@@ -332,11 +332,18 @@ class Backend:
         return out
 
     def loss_grad(self, P, rho_ptr, grad_ptr, kind=0, eps=1e-3, stream=None, want_loss=True):
-        """Loss and gradient of P fields at device addresses rho_ptr -> grad_ptr."""
+        """Loss and gradient of P fields at device addresses rho_ptr -> grad_ptr
+        (grad_ptr None: the losses only)."""
         loss = np.zeros(P, np.float64) if want_loss else None
-        self._check(self.lib.spr_loss_grad(self.h, int(P), ctypes.c_void_p(rho_ptr), ctypes.c_void_p(grad_ptr),
+        self._check(self.lib.spr_loss_grad(self.h, int(P), ctypes.c_void_p(rho_ptr), ctypes.c_void_p(grad_ptr or 0),
                                            _ptr(loss), int(kind), float(eps), ctypes.c_void_p(stream or 0)))
         return loss
+
+    def device_buffers(self):
+        """Device addresses (rho, uint8 supports, amplitude) of the resident population."""
+        r, s, a = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        self._check(self.lib.spr_device_buffers(self.h, ctypes.byref(r), ctypes.byref(s), ctypes.byref(a)))
+        return r.value, s.value, a.value
 
     def refine(self, steps, lr=1e-2, kind=0, eps=1e-3):
         loss = np.zeros(self.P, np.float64)
