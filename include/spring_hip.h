@@ -22,7 +22,7 @@
  *
  * Error model: every function returns 0 on success and a negative code on
  * failure; spr_last_error() returns the message of the last failure on that
- * handle (or of the last failed spr_create when called with NULL). The
+ * handle (or, called with NULL, of the last failed call without a handle). The
  * reference propagates no errors (cuFFT failures are printed and ignored,
  * src/gpu/gpu_context.cu:750-752); this ABI reports them instead.
  *
@@ -69,7 +69,9 @@ int spr_create(spr_ctx** out, int n, int capacity, const float* amp, int device)
 /* Replaces GPU_context::~GPU_context (src/gpu/gpu_context.h:87-137). */
 int spr_destroy(spr_ctx* ctx);
 
-/* Message of the last failure (never NULL). */
+/* Message of the last failure (never NULL). With ctx == NULL: the last failure
+ * of this thread's calls that take no handle (spr_create,
+ * spr_loss_grad_noalloc). */
 const char* spr_last_error(const spr_ctx* ctx);
 
 /* Grid size and capacity of a handle. */
@@ -224,6 +226,34 @@ int spr_crossover(spr_ctx* ctx, int nchild, const spr_cross_desc* desc, const in
  * is requested. */
 int spr_loss_grad(spr_ctx* ctx, int P, const float* rho, float* grad, double* loss, int kind, float eps,
                   void* stream);
+
+/* spr_loss_grad without a handle and without any allocation. No counterpart in
+ * the reference: this entry point is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). It calls no hipMalloc, hipFree or allocating library
+ * routine and never synchronises.
+ *   n, P   grid size (spr_supported_size) and number of fields;
+ *   amp    device, n*n floats: the corner-origin amplitude with the mask
+ *          encoding of spr_create;
+ *   rho    device, P*n*n complex: the densities (read only);
+ *   grad   device, P*n*n complex, not overlapping rho: the gradient of
+ *          spr_loss_grad when want_grad != 0; it is also the intermediate
+ *          field of the passes, so with want_grad == 0 it holds no gradient;
+ *   loss   device, P doubles (FP64 sums), or NULL;
+ *   stream the hipStream_t to queue on (NULL: the null stream); the current
+ *          device is the one used.
+ * Results are identical to spr_loss_grad's (same kernels). Its only scratch
+ * (loss partials per column workgroup, twiddle roots) is a fixed device array
+ * of 1.04 MiB per GPU, held until the process ends, whose address and roots
+ * are set up by the first call on each device (synchronously; make that first
+ * call outside graph capture). The partials limit P * n / C <= 131072 (C =
+ * columns per column-pass workgroup): at most 5461 fields of 384^2, 4096 of
+ * 512^2, 1365 of 768^2, 1024 of 1024^2, 256 of 2048^2; larger counts are
+ * rejected with SPR_ERR_ARG. Every call on one GPU shares that GPU's array:
+ * calls that overlap in time on one GPU (on different streams or host threads)
+ * race on it and must be serialised by the caller. Errors: spr_last_error(NULL). */
+int spr_loss_grad_noalloc(int n, int P, const float* amp, const float* rho, float* grad, double* loss, int kind,
+                          float eps, int want_grad, void* stream);
 
 /* `steps` Adam steps (lr; beta 0.9/0.999, eps 1e-8, moments reset per call)
  * of the loss above on every resident density. loss (optional, host, P

@@ -1059,6 +1059,71 @@ int check_indices(spr_ctx* c, const int32_t* idx, int count, int limit, const ch
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Fixed device scratch of spr_loss_grad_noalloc, the entry point that
+// allocates nothing. Layout: the twiddle roots exp(-2 pi i m/N) of every
+// supported size (NOALLOC_TW_FLOAT2 float2, sizes in SPR_SIZES order), then
+// NOALLOC_PARTIALS doubles of per-workgroup loss partials. Each GPU holds its
+// own copy for the life of the process. The array's name is a host-side
+// stand-in in host code: host code reaches the memory only through the address
+// hipGetSymbolAddress returns for the current device (noalloc_setup).
+// ---------------------------------------------------------------------------
+constexpr int NOALLOC_TW_FLOAT2 = 64 + 128 + 256 + 384 + 512 + 768 + 1024 + 2048;
+constexpr size_t NOALLOC_TW_BYTES = ((NOALLOC_TW_FLOAT2 * sizeof(float2) + 255) / 256) * 256;
+constexpr size_t NOALLOC_PARTIALS = 131072;
+constexpr size_t NOALLOC_BYTES = NOALLOC_TW_BYTES + NOALLOC_PARTIALS * sizeof(double);
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+
+// Offset (float2) of size n's roots in the scratch twiddle region; -1 if n is
+// not supported.
+int noalloc_tw_offset(int n) {
+    int off = 0;
+#define SPR_TWOFF_CASE(N_)         \
+    if (n == N_) return off;       \
+    off += N_;
+    SPR_SIZES(SPR_TWOFF_CASE)
+#undef SPR_TWOFF_CASE
+    return -1;
+}
+
+std::mutex g_noalloc_mu;
+unsigned char* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+float2 g_noalloc_tw_host[NOALLOC_TW_FLOAT2];   // staging of the roots (set-up only)
+
+// Once per device, with that device current: the device address of `scratch`
+// and the upload of every size's twiddle roots (computed in double and rounded
+// to float, as for a handle). Synchronous; later calls only read the address.
+int noalloc_setup(int dev, unsigned char** out) {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(nullptr, SPR_ERR_ARG, "device index out of range");
+    if (!g_noalloc_ptr[dev]) {
+        unsigned char* p = nullptr;
+        hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(nullptr, SPR_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        int off = 0;
+#define SPR_TWFILL_CASE(N_)                                                                      \
+    for (int k = 0; k < N_; ++k) {                                                               \
+        const double ang = -2.0 * M_PI * (double)k / (double)N_;                                 \
+        g_noalloc_tw_host[off + k] = make_float2((float)std::cos(ang), (float)std::sin(ang));    \
+    }                                                                                            \
+    off += N_;
+        SPR_SIZES(SPR_TWFILL_CASE)
+#undef SPR_TWFILL_CASE
+        e = hipMemcpy(p, g_noalloc_tw_host, sizeof(g_noalloc_tw_host), hipMemcpyHostToDevice);
+        if (e != hipSuccess)
+            return fail(nullptr, SPR_ERR_HIP, std::string("upload of the twiddle roots failed: ") + hipGetErrorString(e));
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return SPR_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int spr_supported_size(int n) {
@@ -1862,6 +1927,48 @@ int spr_loss_grad(spr_ctx* c, int P, const float* rho, float* grad, double* loss
         for (int p = 0; p < P; ++p) loss[p] = s[(size_t)p * SUMS_PER_ITEM + 7];
     }
     return harvest_timing(c);
+}
+
+int spr_loss_grad_noalloc(int n, int P, const float* amp, const float* rho, float* grad, double* loss, int kind,
+                          float eps, int want_grad, void* stream) {
+    if (!amp || !rho || !grad) return fail(nullptr, SPR_ERR_ARG, "NULL argument");
+    if (!spr_supported_size(n)) return fail(nullptr, SPR_ERR_ARG, "unsupported pattern size");
+    if (P < 1) return fail(nullptr, SPR_ERR_ARG, "field count must be >= 1");
+    if (kind != LOSS_POISSON && kind != LOSS_AMPLITUDE) return fail(nullptr, SPR_ERR_ARG, "unknown loss kind");
+    const size_t per_field = (size_t)(n / cols_per_wg(n));
+    if ((size_t)P * per_field > NOALLOC_PARTIALS)
+        return fail(nullptr, SPR_ERR_ARG,
+                    "field count exceeds the fixed scratch (" + std::to_string(NOALLOC_PARTIALS / per_field) +
+                        " fields of " + std::to_string(n) + "^2 at most)");
+    const size_t bytes = (size_t)P * n * n * sizeof(float2);
+    const char *r0 = (const char*)rho, *g0 = (const char*)grad;
+    if (r0 < g0 + bytes && g0 < r0 + bytes) return fail(nullptr, SPR_ERR_ARG, "rho and grad overlap");
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(nullptr, SPR_ERR_HIP, std::string("hipGetDevice failed: ") + hipGetErrorString(e));
+    unsigned char* base = nullptr;
+    SPR_TRY(noalloc_setup(dev, &base));
+    // A handle on the stack that borrows the caller's buffers and the fixed
+    // scratch; its launch helpers queue kernels and allocate nothing.
+    spr_ctx c;
+    c.n = n;
+    c.cap = c.P = P;
+    c.device = dev;
+    c.stream = (hipStream_t)stream;
+    c.amp = const_cast<float*>(amp);
+    c.tw = reinterpret_cast<float2*>(base) + noalloc_tw_offset(n);
+    c.partial = reinterpret_cast<double*>(base + NOALLOC_TW_BYTES);
+    const float2* r = (const float2*)rho;
+    float2* g = (float2*)grad;
+    int rc = queue_loss_passes(&c, P, r, g, want_grad ? g : nullptr, kind, eps);
+    if (rc == SPR_OK && loss) {
+        hipLaunchKernelGGL(k_reduce_partials, dim3(P), dim3(256), 0, c.stream, (const double*)c.partial, (int)per_field,
+                           1, loss, 1, 0, P);
+        e = hipGetLastError();
+        if (e != hipSuccess) rc = fail(&c, SPR_ERR_HIP, std::string("loss reduction launch failed: ") + hipGetErrorString(e));
+    }
+    if (rc != SPR_OK) g_create_error = c.err;
+    return rc;
 }
 
 int spr_refine(spr_ctx* c, int steps, float lr, int kind, float eps, double* loss) {

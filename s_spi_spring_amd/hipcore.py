@@ -61,6 +61,7 @@ SIGNATURES = {
     "spr_device_count": (_I, [ctypes.POINTER(_I)]),
     "spr_device_name": (_I, [_I, ctypes.c_char_p, _I]),
     "spr_loss_grad": (_I, [_VP, _I, _VP, _VP, _VP, _I, _F, _VP]),
+    "spr_loss_grad_noalloc": (_I, [_I, _I, _VP, _VP, _VP, _VP, _I, _F, _I, _VP]),
     "spr_refine": (_I, [_VP, _I, _F, _I, _F, _VP]),
     "spr_refine_support": (_I, [_VP, _I, _F, _I, _F, _VP]),
     "spr_batch_create": (_I, [ctypes.POINTER(_VP), _I, _VP, _VP, _VP, _I]),
@@ -148,6 +149,19 @@ def load(path: str = LIB_PATH):
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def loss_grad_noalloc(n, P, amp_ptr, rho_ptr, grad_ptr, loss_ptr=None, kind=0, eps=1e-3, want_grad=True,
+                      stream=None):
+    """spr_loss_grad_noalloc on device addresses: losses (FP64, P entries at
+    loss_ptr, optional) and gradients of P n x n fields, queued on `stream`
+    without any allocation or synchronisation."""
+    lib = load()
+    rc = lib.spr_loss_grad_noalloc(int(n), int(P), ctypes.c_void_p(amp_ptr), ctypes.c_void_p(rho_ptr),
+                                   ctypes.c_void_p(grad_ptr), ctypes.c_void_p(loss_ptr or 0), int(kind), float(eps),
+                                   int(bool(want_grad)), ctypes.c_void_p(stream or 0))
+    if rc != 0:
+        raise SpringHipError(lib.spr_last_error(None).decode())
 
 
 def variant_counts(n):
