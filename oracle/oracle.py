@@ -251,3 +251,73 @@ def rng_uniform(seed, count):
     out = np.zeros(count, dtype=np.float32)
     lib().orc_rng_uniform(ctypes.c_int(seed), ctypes.c_int(count), _p(out))
     return out
+
+
+def crossover_draws(seed, nchild, P, sorted_idx, suppdim, n, crossexp=1.0, crossftin=1.0, crossbalance=0.6,
+                    stream=True):
+    """Random draws of Solver::Crossover (src/solver.cpp:742-801). stream=True:
+    nchild consecutive children from one context seeded `seed`; False: child i
+    from its own context seeded (i+1)*seed. Returns a list of dicts (p1, p2, p3,
+    crossft, crosssizex/y, squaresizex/y, offsetx/y, rotangle, crossmap)."""
+    si = np.ascontiguousarray(sorted_idx, dtype=np.int32)
+    di = np.zeros(11 * nchild, np.int32)
+    df = np.zeros(nchild, np.float32)
+    cap = 4 * (n + 64) * (n + 64)
+    cm = np.zeros(cap, np.int32)
+    cl = np.zeros(nchild, np.int32)
+    args = [int(P), _p(si), ctypes.c_float(suppdim), int(n), ctypes.c_float(crossexp), ctypes.c_float(crossftin),
+            ctypes.c_float(crossbalance), _p(di), _p(df), _p(cm), _p(cl), int(cap)]
+    if stream:
+        rc = lib().orc_crossover_draws_stream(int(nchild), int(seed), *args)
+    else:
+        seeds = np.array([(i + 1) * seed for i in range(nchild)], np.int32)
+        rc = lib().orc_crossover_draws(int(nchild), _p(seeds), *args)
+    if rc != 0:
+        raise RuntimeError("crossmap buffer too small")
+    out, off = [], 0
+    for c in range(nchild):
+        o = di[11 * c:11 * c + 11]
+        out.append(dict(p1=int(o[0]), p2=int(o[1]), p3=int(o[2]), crosssizex=int(o[3]), crosssizey=int(o[4]),
+                        squaresizex=int(o[5]), squaresizey=int(o[6]), offsetx=int(o[7]), offsety=int(o[8]),
+                        crossft=int(o[9]), rotangle=float(df[c]), crossmap=cm[off:off + cl[c]].copy()))
+        off += int(cl[c])
+    return out
+
+
+def _init_out(n, count):
+    return np.zeros((count, n, n), np.complex64), np.zeros((count, n, n), np.int32)
+
+
+def init_disk(amp, seed, count, support_size, itemsize, nitems, exponent, scaling, phase):
+    """Solver::Initialize_rec with a disk support (src/solver.cpp:870-957): `count`
+    individuals drawn in turn from one context RNG seeded `seed`."""
+    a = np.ascontiguousarray(amp, dtype=np.float32)
+    n = a.shape[0]
+    fp = np.array([itemsize[0], itemsize[1], exponent[0], exponent[1], scaling[0], scaling[1], phase], np.float32)
+    ip = np.array(nitems, np.int32)
+    rho, S = _init_out(n, count)
+    lib().orc_init_disk(int(n), _p(a), int(seed), int(count), int(support_size), _p(fp), _p(ip), _p(rho), _p(S))
+    return rho, S
+
+
+def init_ref(ref_rho, ref_S, seed, count, itemsize, nitems, exponent, scaling, phase):
+    """Initialize_rec from a reference reconstruction (src/solver.cpp:961-1058)."""
+    r = np.ascontiguousarray(ref_rho, dtype=np.complex64)
+    s = np.ascontiguousarray(ref_S, dtype=np.int32)
+    n = r.shape[0]
+    fp = np.array([itemsize[0], itemsize[1], exponent[0], exponent[1], scaling[0], scaling[1], phase], np.float32)
+    ip = np.array(nitems, np.int32)
+    rho, S = _init_out(n, count)
+    lib().orc_init_ref(int(n), _p(r), _p(s), int(seed), int(count), _p(fp), _p(ip), _p(rho), _p(S))
+    return rho, S
+
+
+def init_supp(amp, support, seed, count, initfrac, scaling, phase):
+    """Initialize_rec from a support with random Fourier cells (src/solver.cpp:1061-1174)."""
+    a = np.ascontiguousarray(amp, dtype=np.float32)
+    s = np.ascontiguousarray(support, dtype=np.int32)
+    n = a.shape[0]
+    fp = np.array([initfrac, scaling[0], scaling[1], phase], np.float32)
+    rho, S = _init_out(n, count)
+    lib().orc_init_supp(int(n), _p(a), _p(s), int(seed), int(count), _p(fp), _p(rho), _p(S))
+    return rho, S

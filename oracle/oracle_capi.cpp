@@ -164,9 +164,98 @@ extern "C" int orc_crossover_draws(int nchild, const int* seeds, int P, const in
     return 0;
 }
 
+// Draws of `nchild` consecutive children from ONE Rng seeded with `seed`, as the
+// children that one reference context serves in turn (src/solver.cpp:742-801);
+// outputs as orc_crossover_draws.
+extern "C" int orc_crossover_draws_stream(int nchild, int seed, int P, const int* sorted_idx, float suppdim, int n,
+                                          float crossexp, float crossftin, float crossbalance, int* di_out,
+                                          float* df_out, int32_t* cm_out, int* cm_len, int cm_cap) {
+    Rng g(seed);
+    size_t off = 0;
+    for (int c = 0; c < nchild; ++c) {
+        int p1, p2, p3;
+        CrossDesc d{};
+        std::vector<int32_t> cm;
+        crossover_draws(g, P, sorted_idx, suppdim, n, crossexp, crossftin, p1, p2, p3, d, cm, crossbalance);
+        int* o = di_out + 11 * c;
+        o[0] = p1; o[1] = p2; o[2] = p3; o[3] = d.crosssizex; o[4] = d.crosssizey; o[5] = d.squaresizex;
+        o[6] = d.squaresizey; o[7] = d.offsetx; o[8] = d.offsety; o[9] = d.crossft; o[10] = d.crosssym;
+        df_out[c] = d.rotangle;
+        cm_len[c] = (int)cm.size();
+        if (off + cm.size() > (size_t)cm_cap) return -1;
+        std::copy(cm.begin(), cm.end(), cm_out + off);
+        off += cm.size();
+    }
+    return 0;
+}
+
 // First `count` draws u ~ U(0,1) of a Context RNG seeded with `seed` (src/context.h:13-18).
 extern "C" int orc_rng_uniform(int seed, int count, float* out) {
     Rng g(seed);
     for (int i = 0; i < count; ++i) out[i] = g.rand();
+    return 0;
+}
+
+// Population initialisation: `count` consecutive individuals drawn from ONE Rng
+// seeded `seed` (one reference context serving them in turn), fields in
+// rho_out (count*n*n complex64) and S_out (count*n*n int32).
+// fp = {itemsize0, itemsize1, exponent0, exponent1, scaling0, scaling1, phase}, ip = {nitems0, nitems1}.
+extern "C" int orc_init_disk(int n, const float* amp, int seed, int count, int support_size, const float* fp,
+                             const int* ip, float* rho_out, int32_t* S_out) {
+    Plan pl(n);
+    Rng g(seed);
+    const size_t np = (size_t)n * n;
+    for (int r = 0; r < count; ++r)
+        init_disk(g, pl, amp, support_size, fp, ip, fp + 2, fp + 4, fp[6],
+                  reinterpret_cast<cplx<float>*>(rho_out) + r * np, S_out + r * np);
+    return 0;
+}
+extern "C" int orc_init_ref(int n, const float* ref_rho, const int32_t* ref_S, int seed, int count, const float* fp,
+                            const int* ip, float* rho_out, int32_t* S_out) {
+    Rng g(seed);
+    const size_t np = (size_t)n * n;
+    for (int r = 0; r < count; ++r)
+        init_ref(g, n, reinterpret_cast<const cplx<float>*>(ref_rho), ref_S, fp, ip, fp + 2, fp + 4, fp[6],
+                 reinterpret_cast<cplx<float>*>(rho_out) + r * np, S_out + r * np);
+    return 0;
+}
+// fp = {initfrac, scaling0, scaling1, phase}
+extern "C" int orc_init_supp(int n, const float* amp, const int32_t* support, int seed, int count, const float* fp,
+                             float* rho_out, int32_t* S_out) {
+    Plan pl(n);
+    Rng g(seed);
+    const size_t np = (size_t)n * n;
+    for (int r = 0; r < count; ++r)
+        init_supp(g, pl, amp, support, fp[0], fp + 1, fp[3], reinterpret_cast<cplx<float>*>(rho_out) + r * np,
+                  S_out + r * np);
+    return 0;
+}
+
+// Stateful context RNG handles: one stream serving several calls in turn, as a
+// reference context serves Initialize_rec and then Crossover draws.
+extern "C" void* orc_rng_create(int seed) { return new Rng(seed); }
+extern "C" void orc_rng_destroy(void* g) { delete static_cast<Rng*>(g); }
+extern "C" int orc_init_disk_rng(void* g, int n, const float* amp, int support_size, const float* fp, const int* ip,
+                                 float* rho_out, int32_t* S_out) {
+    Plan pl(n);
+    init_disk(*static_cast<Rng*>(g), pl, amp, support_size, fp, ip, fp + 2, fp + 4, fp[6],
+              reinterpret_cast<cplx<float>*>(rho_out), S_out);
+    return 0;
+}
+extern "C" int orc_crossover_draws_rng(void* g, int P, const int* sorted_idx, float suppdim, int n, float crossexp,
+                                       float crossftin, float crossbalance, int* di_out, float* df_out,
+                                       int32_t* cm_out, int* cm_len, int cm_cap) {
+    int p1, p2, p3;
+    CrossDesc d{};
+    std::vector<int32_t> cm;
+    crossover_draws(*static_cast<Rng*>(g), P, sorted_idx, suppdim, n, crossexp, crossftin, p1, p2, p3, d, cm,
+                    crossbalance);
+    di_out[0] = p1; di_out[1] = p2; di_out[2] = p3; di_out[3] = d.crosssizex; di_out[4] = d.crosssizey;
+    di_out[5] = d.squaresizex; di_out[6] = d.squaresizey; di_out[7] = d.offsetx; di_out[8] = d.offsety;
+    di_out[9] = d.crossft; di_out[10] = d.crosssym;
+    df_out[0] = d.rotangle;
+    *cm_len = (int)cm.size();
+    if (cm.size() > (size_t)cm_cap) return -1;
+    std::copy(cm.begin(), cm.end(), cm_out);
     return 0;
 }

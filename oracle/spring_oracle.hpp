@@ -721,6 +721,178 @@ inline void crossover_draws(Rng& g, int P, const int* sorted_idx, float suppdim,
         if (g.rand(0, 1) < crossbalance) crossmap[i] = 1;
 }
 
+// ----------------------------------------------------------------------------
+// Population initialisation (Solver::Initialize_rec, src/solver.cpp:870-1174).
+// Host arithmetic in the reference's types: FLOAT = float, COMPLEX =
+// std::complex<float>; expressions that the reference forms in double (a float
+// times a double literal, std::pow(float, int)) are formed in double here and
+// rounded where the reference assigns to a FLOAT. The FT/IFT the reference runs
+// on the device are this file's FP32 transforms. Each call draws one
+// individual from `g` in the reference's order.
+// ----------------------------------------------------------------------------
+
+// Mean-preserving scale of an initial density: the reference's norm loop
+// (src/solver.cpp:943-954, 1155-1166) over the measured pixels of amp.
+inline float init_norm_factor(const cplx<float>* F, const float* amp, size_t np) {
+    float norm = 0, norm_new = 0;
+    for (size_t i = 0; i < np; ++i)
+        if (amp[i] >= 0) {
+            norm += std::pow(std::abs(F[i]), 2);   // std::pow(float, int): double, added in double
+            norm_new += amp[i] * amp[i];
+        }
+    return std::sqrt(norm_new / norm);
+}
+
+// Initialize_rec with a disk support of diameter support_size centred at
+// (n/2, n/2) and nitems random projected spheres (src/solver.cpp:870-957).
+inline void init_disk(Rng& g, const Plan& pl, const float* amp, int support_size, const float itemsize[2],
+                      const int nitems_in[2], const float exponent_in[2], const float scaling_in[2], float phase_in,
+                      cplx<float>* rho, int32_t* S) {
+    const int n = pl.n;
+    const size_t np = (size_t)n * n;
+    const int nitems = (int)g.rand((float)nitems_in[0], (float)(nitems_in[1] + 0.99));
+    const float exponent = g.rand(exponent_in[0], exponent_in[1]);
+    const float scaling = g.rand(scaling_in[0], scaling_in[1]);
+    std::vector<float> sig(nitems), cx(nitems), cy(nitems), ph(nitems);
+    for (int i = 0; i < nitems; ++i) {
+        const float s2 = g.rand((float)std::pow(itemsize[0], 2), (float)std::pow(itemsize[1], 2));
+        sig[i] = (float)(std::sqrt(s2) / 2.);
+        cx[i] = g.rand(sig[i], (float)support_size - sig[i]);
+        cy[i] = g.rand(sig[i], (float)support_size - sig[i]);
+        ph[i] = g.rand((float)(-phase_in * M_PI), (float)(phase_in * M_PI));
+    }
+    std::fill(rho, rho + np, cplx<float>(0, 0));
+    std::fill(S, S + np, 0);
+    for (int iy = 0; iy < support_size; ++iy)
+        for (int ix = 0; ix < support_size; ++ix) {
+            const double dx = ix - support_size / 2., dy = iy - support_size / 2.;
+            const float supprad = (float)std::sqrt(dx * dx + dy * dy);
+            if (!(supprad <= support_size / 2.)) continue;
+            const size_t idx = (size_t)(iy + n / 2 - support_size / 2) * n + (ix + n / 2 - support_size / 2);
+            S[idx] = 1;
+            for (int i = 0; i < nitems; ++i) {
+                const float rad = std::sqrt((ix - cx[i]) * (ix - cx[i]) + (iy - cy[i]) * (iy - cy[i]));
+                if (rad < sig[i]) {
+                    const float val = (float)(2. * sig[i] * std::sin(std::acos(rad / sig[i])));
+                    rho[idx] += std::polar(val, ph[i]);
+                }
+            }
+            const float mod = std::abs(rho[idx]);
+            if (mod > 0) rho[idx] = rho[idx] / mod * std::pow(mod, exponent);
+        }
+    std::vector<cplx<float>> F(rho, rho + np);
+    ft(pl, F.data());
+    const float f = init_norm_factor(F.data(), amp, np) * scaling;
+    for (size_t i = 0; i < np; ++i) rho[i] *= f;
+}
+
+// Initialize_rec from a reference reconstruction: spheres centred on support
+// pixels picked by rank, the reference's support, the sum's modulus scaled
+// (src/solver.cpp:961-1058). The pixel of rank k is found as the reference
+// does: the first index j at which k support pixels precede j.
+inline void init_ref(Rng& g, int n, const cplx<float>* ref_rho, const int32_t* ref_S, const float itemsize[2],
+                     const int nitems_in[2], const float exponent_in[2], const float scaling_in[2], float phase_in,
+                     cplx<float>* rho, int32_t* S) {
+    const size_t np = (size_t)n * n;
+    const int nitems = (int)g.rand((float)nitems_in[0], (float)(nitems_in[1] + 0.99));
+    const float exponent = g.rand(exponent_in[0], exponent_in[1]);
+    const float scaling = g.rand(scaling_in[0], scaling_in[1]);
+    int supppix = 0;
+    cplx<float> sum = 0;
+    for (size_t i = 0; i < np; ++i) {
+        sum += ref_rho[i];
+        if (ref_S[i] > 0) supppix += 1;
+    }
+    const float newnorm = std::abs(sum);
+    std::vector<float> sig(nitems), cx(nitems), cy(nitems), ph(nitems);
+    for (int i = 0; i < nitems; ++i) {
+        const float s2 = g.rand((float)std::pow(itemsize[0], 2), (float)std::pow(itemsize[1], 2));
+        sig[i] = (float)(std::sqrt(s2) / 2.);
+        const int target = (int)g.rand(0.f, (float)supppix);
+        int counter = 0;
+        size_t index = 0;
+        for (size_t j = 0; j < np; ++j) {
+            if (counter == target) {
+                index = j;
+                break;
+            }
+            if (ref_S[j] > 0) counter += 1;
+        }
+        cx[i] = (float)(index % n);
+        cy[i] = (float)(index / n);
+        ph[i] = g.rand((float)(-phase_in * M_PI), (float)(phase_in * M_PI));
+    }
+    std::fill(rho, rho + np, cplx<float>(0, 0));
+    for (int iy = 0; iy < n; ++iy)
+        for (int ix = 0; ix < n; ++ix) {
+            const size_t idx = (size_t)iy * n + ix;
+            if (ref_S[idx] <= 0) continue;
+            for (int i = 0; i < nitems; ++i) {
+                const float rad = std::sqrt((ix - cx[i]) * (ix - cx[i]) + (iy - cy[i]) * (iy - cy[i]));
+                if (rad < sig[i]) {
+                    const float val = (float)(2. * sig[i] * std::sin(std::acos(rad / sig[i])));
+                    rho[idx] += std::polar(val, ph[i]);
+                }
+            }
+            const float mod = std::abs(rho[idx]);
+            if (mod > 0) rho[idx] = rho[idx] / mod * std::pow(mod, exponent);
+        }
+    cplx<float> oldsum = 0;
+    for (size_t i = 0; i < np; ++i) oldsum += rho[i];
+    const float oldnorm = std::abs(oldsum);
+    for (size_t i = 0; i < np; ++i) rho[i] *= newnorm / oldnorm * scaling;
+    std::copy(ref_S, ref_S + np, S);
+}
+
+// Initialize_rec from a support with random Fourier cells: FT of the support,
+// each selected cell of a rotated square grid scaled by U(0.5, 2) and phased
+// by U(-pi phase, pi phase), IFT, mean-preserving scale (src/solver.cpp:1061-1174).
+// The cell index is evaluated in float as the reference's host code does.
+inline void init_supp(Rng& g, const Plan& pl, const float* amp, const int32_t* support, float initfrac,
+                      const float scaling_in[2], float phase_in, cplx<float>* rho, int32_t* S) {
+    const int n = pl.n;
+    const size_t np = (size_t)n * n;
+    const float normfactor = g.rand(scaling_in[0], scaling_in[1]);
+    const int sqx = (int)g.rand(1.f, (float)((float)n / 16.));
+    const int sqy = (int)g.rand(1.f, (float)((float)n / 16.));
+    const int csx = (n + sqx - 1) / sqx + 1;
+    const int csy = (n + sqy - 1) / sqy + 1;
+    const int offx = (int)g.rand(0.f, (float)sqx);
+    const int offy = (int)g.rand(0.f, (float)sqy);
+    const float rot = g.rand(0.f, (float)M_PI);
+    const size_t cells = (size_t)csx * csy;
+    std::vector<int32_t> map(cells, 0);
+    std::vector<float> scale(cells, 0), phase(cells, 0);
+    for (size_t i = 0; i < cells; ++i)
+        if (g.rand(0.f, 1.f) < initfrac) {
+            map[i] = 1;
+            scale[i] = g.rand(0.5f, 2.f);
+            phase[i] = g.rand((float)(-M_PI * phase_in), (float)(M_PI * phase_in));
+        }
+    std::vector<cplx<float>> d(np);
+    for (size_t i = 0; i < np; ++i) d[i] = cplx<float>((float)support[i], 0.f);
+    ft(pl, d.data());
+    const cplx<float> imag(0.0f, 1.0f);
+    for (size_t i = 0; i < np; ++i) {
+        int x = (int)(i % n), y = (int)(i / n);
+        if (x >= n / 2) x -= n;
+        if (y >= n / 2) y -= n;
+        float radius = std::sqrt((float)(x * x + y * y));
+        if (radius >= n / 2.) radius = (float)(n / 2 - 1);
+        const float angle = std::atan2((float)y, (float)x) + rot;
+        x = (int)(std::cos(angle) * radius) + n / 2;
+        y = (int)(std::sin(angle) * radius) + n / 2;
+        const int ci = (x + offx) / sqx + ((y + offy) / sqy) * csx;
+        if (map[ci] == 1) d[i] = d[i] * scale[ci] * std::exp(imag * phase[ci]);
+    }
+    ift(pl, d.data());
+    std::vector<cplx<float>> F(d);
+    ft(pl, F.data());
+    const float f = init_norm_factor(F.data(), amp, np) * normfactor;
+    for (size_t i = 0; i < np; ++i) rho[i] = d[i] * f;
+    std::copy(support, support + np, S);
+}
+
 // Solver::set_boundaries + Solver::prepare (src/solver.cpp:209-289, 173-206):
 // intensity (centre at `center`, -1 masked, -2 saturated) -> corner-origin amplitude.
 inline void prepare_pattern(const float* intensity, int n, int cx, int cy, bool use_bounds, float* amp_out) {

@@ -46,6 +46,50 @@ def test_initialize_pop_properties(pr, orc):
         assert rel_l2(a.get_density(), b.get_density()) < 1e-6
 
 
+@pytest.mark.parametrize("n,nthreads", [(128, 1), (128, 6), (512, 1), (512, 6)])
+def test_initialize_pop_matches_oracle(pr, orc, n, nthreads):
+    """InitializePop (disk support, src/solver.cpp:870-957) and
+    InitializePopSupp (random Fourier cells, :1061-1174) against the oracle's
+    restatements: supports bit-exact, densities within 1e-5 (the device FFT of
+    the initial density enters through the energy factor, and through the
+    cells' transform pair in the support variant). Individual r draws from
+    context r % nthreads: one stream for the whole population (nthreads 1) or
+    one per individual."""
+    P = 6
+    I, _ = synth.polygon_pattern(seed=3, n=n) if n <= 128 else synth.ellipsoids_pattern(seed=3, n=n)
+    S = pr.Solver(I, reality=0.0, realpen=0.0, nthreads=nthreads, use_bounds=False, gpus=[0], seed=17)
+    amp = S.get_pattern()
+    s = n // 3
+    args = dict(itemsize=[s * 0.2, s * 0.9], nitems=[2, 8], exponent=[0.4, 0.9], scaling=[0.5, 1.5],
+                phaserange=0.2)
+    recs = S.InitializePop(popsize=P, suppsize=s, suppcut=0, **args)
+    fp = (args["itemsize"], args["nitems"], args["exponent"], args["scaling"], args["phaserange"])
+
+    def oracle_pop(draw):
+        """draw(seed, count) -> (rho, S) of count individuals from one stream."""
+        if nthreads == 1:
+            return draw(17, P)
+        parts = [draw((i + 1) * 17, 1) for i in range(P)]
+        return np.concatenate([p[0] for p in parts]), np.concatenate([p[1] for p in parts])
+
+    want, wS = oracle_pop(lambda seed, count: orc.init_disk(amp, seed, count, s, *fp))
+    for i, rec in enumerate(recs):
+        assert np.array_equal(rec.get_support(), wS[i])
+        e = rel_l2(rec.get_density(), want[i])
+        print(f"n={n} nthreads={nthreads} disk individual {i}: rel-L2 {e:.2e}", flush=True)
+        assert e < 1e-5
+    # the support variant, from a fresh Solver (fresh streams)
+    S2 = pr.Solver(I, reality=0.0, realpen=0.0, nthreads=nthreads, use_bounds=False, gpus=[0], seed=17)
+    ref_S = synth.disk_support(n, s)
+    recs = S2.InitializePopSupp(popsize=P, reference=ref_S, fraction=0.4, scaling=[0.5, 1.5], phaserange=0.3)
+    want, wS = oracle_pop(lambda seed, count: orc.init_supp(amp, ref_S, seed, count, 0.4, (0.5, 1.5), 0.3))
+    for i, rec in enumerate(recs):
+        assert np.array_equal(rec.get_support(), wS[i])
+        e = rel_l2(rec.get_density(), want[i])
+        print(f"n={n} nthreads={nthreads} support individual {i}: rel-L2 {e:.2e}", flush=True)
+        assert e < 1e-5
+
+
 def test_sequence_matches_oracle(pr, orc):
     n, P = 128, 5
     S, I, recs = population(pr, n, P)
@@ -80,7 +124,7 @@ def test_sequence_matches_oracle(pr, orc):
     assert one is recs[0]
 
 
-@pytest.mark.parametrize("n", [128, 384])
+@pytest.mark.parametrize("n", [128, 384, 512, 1024])
 def test_reshift_matches_oracle(pr, orc, n):
     P = 5
     S, I, recs = population(pr, n, P)
@@ -128,9 +172,16 @@ def test_average_and_average_module(pr, orc):
             assert rel_l2(m, wm) < 1e-6
 
 
-@pytest.mark.parametrize("n,crossft,use_average", [(128, 1.0, 0.0), (128, 0.0, 0.0), (128, 1.0, 0.5), (384, 1.0, 0.0)])
-def test_crossover_matches_oracle(pr, orc, n, crossft, use_average):
-    P = 6
+@pytest.mark.parametrize("n,crossft,use_average,crosssym,crossexp", [
+    (128, 1.0, 0.0, False, 1.0), (128, 0.0, 0.0, False, 1.0), (128, 1.0, 0.5, False, 1.0), (384, 1.0, 0.0, False, 1.0),
+    (512, 1.0, -0.5, True, 2.0), (512, 0.5, 0.5, True, 2.0), (1024, 1.0, -0.5, False, 2.0)])
+def test_crossover_matches_oracle(pr, orc, n, crossft, use_average, crosssym, crossexp):
+    """Every child of Solver.Crossover against the oracle's child from the same
+    parents and draws: supports bit-exact, densities within 1e-5. Covers the
+    crossover's parameter space: real-space (crossft 0) and mixed (0.5)
+    children, the use_average > 0 and < 0 modulus rules, the symmetric cell
+    fold (crosssym) and rank-biased parent choice (crossexp 2)."""
+    P = 6 if n <= 512 else 4
     S, I, recs = population(pr, n, P, nthreads=P)
     S.set_sequence([("HIO", [5.0, 0.9]), ("ER", [3.0])])
     S.Sequence(recs)
@@ -140,7 +191,7 @@ def test_crossover_matches_oracle(pr, orc, n, crossft, use_average):
     amp_avg = S.GetAverageModule(recs, ft=True, fraction=0.66)
     # a fresh solver so that child i draws from a fresh stream seeded (i+1)*17
     S2 = pr.Solver(I, reality=0.0, realpen=0.0, nthreads=P, use_bounds=False, gpus=[0], seed=17)
-    children = S2.Crossover(recs, best, avg, 0.6, 0.4, crossexp=1.0, crossft=crossft, crosssym=False,
+    children = S2.Crossover(recs, best, avg, 0.6, 0.4, crossexp=crossexp, crossft=crossft, crosssym=crosssym,
                             usebest=use_average, update_support=True)
     errs = np.array([r.get_error() for r in recs], np.float32)
     sorted_idx = np.array(sorted(range(P), key=lambda i: errs[i]), np.int32)
@@ -148,9 +199,10 @@ def test_crossover_matches_oracle(pr, orc, n, crossft, use_average):
     suppdim = np.float32(min(max(suppdim, 4.0), n / 4.0))
     for i, child in enumerate(children):
         seeds = np.array([(i + 1) * 17], np.int32)
-        draws = orc_crossover_draws(orc, seeds, P, sorted_idx, suppdim, n, 1.0, crossft, 0.6)
+        draws = orc_crossover_draws(orc, seeds, P, sorted_idx, suppdim, n, crossexp, crossft, 0.6)
         p1, p2, p3, desc, cmap = draws
-        desc.update(crossdiff=0.4, use_average=use_average, crosssym=0)
+        desc.update(crossdiff=0.4, use_average=use_average, crosssym=int(crosssym))
+        print(f"n={n} child {i}: parents {i},{p1},{p2},{p3} crossft={desc['crossft']}", flush=True)
         parents = [(recs[j].get_density(), recs[j].get_support()) for j in (i, p1, p2, p3)]
         wd, ws = orc.cross_child(amp, amp_avg, parents, avg.get_density(), cmap, desc, 0.0, 2.0, 0.0, True)
         assert np.array_equal(child.get_support(), ws)

@@ -71,3 +71,42 @@ def test_set_sequence_skips_unknown(core, capfd):
     out = capfd.readouterr().out
     assert "FOO [ 1 ]  NOT RECOGNIZED. Skipping." in out
     assert "ER [ 1 2 ]  NOT RECOGNIZED. Skipping." in out
+
+
+def test_set_sequence_rejects_bad_grad_and_clamps_negative_counts(core):
+    S = core.phase_retrieval.Solver(np.ones((64, 64), np.float32), gpus=[0], nthreads=1)
+    with pytest.raises(Exception, match="GRAD"):
+        S.set_sequence([("GRAD", [10.0, 0.01, 5.0])])
+    with pytest.raises(Exception, match="GRAD"):
+        S.set_sequence([("GRAD", [-1.0, 0.01])])
+    S.set_sequence([("ER", [-3.0]), ("GRAD", [10.0, 0.01, 1.0])])   # accepted
+
+
+def test_multiple_devices_warn(core, capfd):
+    core.phase_retrieval.Solver(np.ones((64, 64), np.float32), gpus=[0, 1], nthreads=1)
+    assert "device 0 only" in capfd.readouterr().err
+
+
+@pytest.mark.parametrize("nthreads", [1, 4])
+def test_initialize_pop_ref_matches_oracle(core, orc, nthreads):
+    """Solver::Initialize_rec from a reference reconstruction (src/solver.cpp:
+    961-1058) is host arithmetic only: the Solver's individuals equal the
+    oracle's bit for bit, individual r drawing from context r % nthreads."""
+    n, P = 128, 4
+    I, _ = synth.polygon_pattern(seed=5, n=n)
+    S = core.phase_retrieval.Solver(I, gpus=[0], nthreads=nthreads, seed=17)
+    r = np.random.default_rng(9)
+    ref_S = synth.disk_support(n, 40)
+    ref = ((r.standard_normal((n, n)) + 1j * r.standard_normal((n, n))) * ref_S).astype(np.complex64)
+    refrec = core.phase_retrieval.Reconstruction(ref, ref_S)
+    args = dict(itemsize=[4.0, 18.0], nitems=[2, 6], exponent=[0.5, 1.0], scaling=[0.5, 1.5], phaserange=0.3)
+    recs = S.InitializePopRef(popsize=P, reference=refrec, suppcut=0, **args)
+    fp = (args["itemsize"], args["nitems"], args["exponent"], args["scaling"], args["phaserange"])
+    if nthreads == 1:
+        want, wS = orc.init_ref(ref, ref_S, 17, P, *fp)
+    else:
+        pairs = [orc.init_ref(ref, ref_S, (i + 1) * 17, 1, *fp) for i in range(P)]
+        want, wS = np.concatenate([p[0] for p in pairs]), np.concatenate([p[1] for p in pairs])
+    for i, rec in enumerate(recs):
+        assert np.array_equal(rec.get_support(), wS[i])
+        assert np.array_equal(rec.get_density(), want[i])
