@@ -1,8 +1,8 @@
-"""The LDS bank model (s_spi_spring_amd/lds_bank_model.py) against measured
+"""The LDS bank model (tests/lds_bank_model.py) against measured
 SQ_LDS_BANK_CONFLICT counts per wave on MI355X (rocprofv3 --pmc runs of
 bench.py, recorded in profiles/iteration_k_col_k_row_512_pop64_raar.md and
 profiles/iteration_block_k_iter_fused_128_pop256_raar.md)."""
-from s_spi_spring_amd import lds_bank_model as m
+import lds_bank_model as m
 
 PLAN_512 = (8, 8, 8)
 
