@@ -185,12 +185,12 @@ def measured_traffic(kernel_name, log):
     return None if k is None else k["hbm_bytes"]
 
 
-def time_to_rf_study(seeds, generations, target, log, threshold=None, groups=()):
-    """Time to best E_F < target of the plain and the gradient schedules over
-    `seeds` seeded 512^2 ellipsoid patterns (population 64); with `groups`, also
-    the plain schedule run k individuals per launch for each k (the reference's
-    structure: each of its k GPU contexts runs one individual at a time). One
-    JSON line."""
+def time_to_rf_runs(seeds, generations, target, log, threshold=None, groups=(), group_seeds=None):
+    """Runs of the time-to-R_F study: the plain and the gradient schedules
+    (whole population per launch) over `seeds` seeded 512^2 ellipsoid patterns
+    (population 64), and for each k in `groups` the plain schedule run k
+    individuals per launch (the reference's structure: each of its k GPU
+    contexts runs one individual at a time) over the first `group_seeds` seeds."""
     from s_spi_spring_amd import workloads
     log("warm-up run (not recorded)")
     workloads.time_to_rf(10_000, "plain", target=target, generations=0)
@@ -198,23 +198,60 @@ def time_to_rf_study(seeds, generations, target, log, threshold=None, groups=())
     runs = {name: [] for name, _, _ in legs}
     for seed in range(seeds):
         for name, form, k in legs:
+            if k and group_seeds is not None and seed >= group_seeds:
+                continue
             r = workloads.time_to_rf(seed, form, target=target, generations=generations, threshold=threshold,
                                      max_batch=k)
             runs[name].append(r)
             log(f"seed {seed:2d} {name:5s}: t={r['time_to_target']} s gen={r['generation']} "
                 f"E_F={r['fourier_error']:.4f} (floor {r['noise_floor']:.4f}) real-space={r['real_space_error']:.4f} "
                 f"wall={r['wall']:.1f} s history={r['history']}")
+    return legs, runs
 
-    def summary(rs):
-        t = [r["time_to_target"] for r in rs if r["time_to_target"] is not None]
-        return {"reached": len(t), "runs": len(rs),
-                "median_time_s": float(np.median(t)) if t else None,
-                "mean_time_s": float(np.mean(t)) if t else None,
-                "median_generations": float(np.median([r["generation"] for r in rs])),
-                "median_fourier_error": float(np.median([r["fourier_error"] for r in rs])),
-                "median_real_space_error": float(np.median([r["real_space_error"] for r in rs])),
-                "real_space_error_q25_q75": [float(np.percentile([r["real_space_error"] for r in rs], q))
-                                             for q in (25, 75)]}
+
+def ttr_summary(rs):
+    t = [r["time_to_target"] for r in rs if r["time_to_target"] is not None]
+    rse = [r["real_space_error"] for r in rs]
+    return {"reached": len(t), "runs": len(rs),
+            "median_time_s": float(np.median(t)) if t else None,
+            "mean_time_s": float(np.mean(t)) if t else None,
+            "median_generations": float(np.median([r["generation"] for r in rs])),
+            "median_fourier_error": float(np.median([r["fourier_error"] for r in rs])),
+            "median_real_space_error": float(np.median(rse)),
+            "real_space_error_q25_q75": [float(np.percentile(rse, q)) for q in (25, 75)]}
+
+
+def _ratio(a, b):
+    return a / b if a and b else None
+
+
+def time_to_rf_fields(seeds, generations, target, log, group_seeds=8):
+    """The time-to-R_F half of the headline metric for the default bench line:
+    batched+gradient against batched plain over `seeds` seeds (the gradient
+    operator's gain), and the plain schedule with 8 individuals per launch (the
+    reference's default of 8 GPU contexts) over `group_seeds` seeds."""
+    from s_spi_spring_amd import workloads
+    legs, runs = time_to_rf_runs(seeds, generations, target, log, groups=(8,), group_seeds=group_seeds)
+    sp, sg, s8 = ttr_summary(runs["plain"]), ttr_summary(runs["grad"]), ttr_summary(runs["plain_groups_of_8"])
+    return {"target_fourier_error": target, "seeds": seeds, "generations_max": generations,
+            "workload": "ellipsoids 512^2, 1e9 photons, population 64, RAAR400+HIO100+ER50 per generation in "
+                        "25-iteration blocks with shrink-wrap; gradient form: 10 support-projected Adam steps "
+                        "on the Poisson likelihood after each RAAR block",
+            "batched_grad": sg, "batched_plain": sp, "plain_8_per_launch": dict(s8, seeds=group_seeds),
+            "speedup_grad_vs_batched_plain": _ratio(sp["median_time_s"], sg["median_time_s"]),
+            "speedup_grad_vs_8_per_launch": _ratio(s8["median_time_s"], sg["median_time_s"]),
+            "median_noise_floor": float(np.median([r["noise_floor"] for r in runs["plain"]])),
+            "sw_threshold": workloads.SW_THRESHOLD_512}
+
+
+def time_to_rf_study(seeds, generations, target, log, threshold=None, groups=()):
+    """Time to best E_F < target of the plain and the gradient schedules over
+    `seeds` seeded 512^2 ellipsoid patterns (population 64); with `groups`, also
+    the plain schedule run k individuals per launch. One JSON line."""
+    from s_spi_spring_amd import workloads
+    legs, runs = time_to_rf_runs(seeds, generations, target, log, threshold, groups)
+    summary = ttr_summary
+
     sp, sg = summary(runs["plain"]), summary(runs["grad"])
     speedup = (sp["median_time_s"] / sg["median_time_s"]) if sp["median_time_s"] and sg["median_time_s"] else None
     grouped = {}
@@ -349,6 +386,10 @@ def main():
     ap.add_argument("--plain-groups", default="", metavar="K,K",
                     help="time-to-rf: also run the plain schedule K individuals per launch for each K")
     ap.add_argument("--generations", type=int, default=10)
+    ap.add_argument("--ttr-seeds", type=int, default=32,
+                    help="default run: seeds of the time-to-R_F fields (0 leaves them out)")
+    ap.add_argument("--ttr-group-seeds", type=int, default=8,
+                    help="default run: seeds of the 8-individuals-per-launch time-to-R_F leg")
     ap.add_argument("--sw-threshold", type=float, default=None,
                     help="shrink-wrap threshold of the time-to-R_F schedules (default 0.03)")
     ap.add_argument("--differentiation", type=int, default=0, metavar="SEEDS",
@@ -404,6 +445,12 @@ def main():
     be.set_population(rho, supps)
     if args.calibrate:
         copy_calibration(args.calibrate, log)
+    # Untimed pre-warm of about half a second, so that the clocks have ramped up
+    # whatever --warmup is, then the requested warmup steps.
+    t_pre = time.perf_counter()
+    while time.perf_counter() - t_pre < 0.5:
+        be.iterate_async(args.alg, args.block, args.beta, 0.0)
+        be.sync()
     log("warmup")
     for _ in range(args.warmup):
         be.iterate_async(args.alg, args.block, args.beta, 0.0)
@@ -467,6 +514,11 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(amp, rho, supps, args.alg, args.beta, args.cpu_iters, log)
+    cvars = be.variants()
+    be.close()
+    ttr = None
+    if args.ttr_seeds > 0 and n == 512 and P == 64:
+        ttr = time_to_rf_fields(args.ttr_seeds, args.generations, args.target, log, args.ttr_group_seeds)
 
     out = {
         "metric": METRIC,
@@ -484,13 +536,13 @@ def main():
         "config": {"workload": f"{'ellipsoids' if n >= 256 else 'polygon'} {n}^2 pattern, population {P}, {args.alg}(beta={args.beta}) "
                                f"blocks of {args.block} iterations, population resident in HBM",
                    "n": n, "population": P, "alg": args.alg, "iterations_per_step": args.block,
-                   "kernel_variants": dict(zip(("column", "row"), be.variants())),
+                   "kernel_variants": dict(zip(("column", "row"), cvars)),
                    "chunk_individuals": min(P, chunk), "on_chip_blocks": fused},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "time_to_rf": ttr,
     }
     print(json.dumps(out), flush=True)
-    be.close()
 
 
 if __name__ == "__main__":

@@ -1,11 +1,14 @@
 """Differentiable masked Fourier-magnitude loss for PyTorch-ROCm.
 
 FourierLoss is a torch.autograd.Function over the hand-written HIP kernels
-behind spr_loss_grad (include/spring_hip.h). When rho needs a gradient, one call
-in forward computes the per-field loss (FP64 sums) and its gradient through the
-adjoint FFT, and backward scales the saved gradient by the incoming one; when it
-does not (torch.no_grad, or a tensor without requires_grad), forward runs the
-loss passes only. Losses (kind):
+behind spr_loss_grad_noalloc (include/spring_hip.h), queued on PyTorch's current
+stream without allocation or synchronisation: the per-field losses (FP64 sums)
+stay on the device. When rho needs a gradient, forward also computes it through
+the adjoint FFT and backward scales the saved gradient by the incoming one;
+when it does not (torch.no_grad, or a tensor without requires_grad), forward
+runs the loss passes only. The entry point's fixed scratch is shared by all its
+calls on one GPU, so losses evaluated concurrently on several streams of one
+GPU must be ordered by the caller. Losses (kind):
 
   "poisson"    L = sum_{A>=0} |F|^2 - I log(|F|^2 + eps),  I = A^2 (counts)
   "amplitude"  L = sum_{A>=0} (|F| - A)^2
@@ -17,6 +20,8 @@ complex tensors (dL/dRe + i dL/dIm), so torch.optim works on rho directly.
 from __future__ import annotations
 
 import torch
+
+from . import hipcore
 
 KINDS = {"poisson": 0, "amplitude": 1}
 
@@ -32,14 +37,17 @@ class FourierLoss(torch.autograd.Function):
         P = rho.shape[0]
         if tuple(rho.shape[1:]) != (backend.n, backend.n):
             raise ValueError("field size differs from the backend's pattern")
-        stream = torch.cuda.current_stream(rho.device).cuda_stream
-        if ctx.needs_input_grad[0]:
-            grad = torch.empty_like(rho)
-            loss = backend.loss_grad(P, rho.data_ptr(), grad.data_ptr(), KINDS[kind], eps, stream)
+        want_grad = bool(ctx.needs_input_grad[0])
+        grad = torch.empty_like(rho)          # the gradient, or the passes' workspace
+        loss = torch.empty(P, dtype=torch.float64, device=rho.device)
+        amp = backend.device_buffers()[2]
+        with torch.cuda.device(rho.device):
+            stream = torch.cuda.current_stream(rho.device).cuda_stream
+            hipcore.loss_grad_noalloc(backend.n, P, amp, rho.data_ptr(), grad.data_ptr(), loss.data_ptr(),
+                                      KINDS[kind], eps, want_grad, stream)
+        if want_grad:
             ctx.save_for_backward(grad)
-        else:
-            loss = backend.loss_grad(P, rho.data_ptr(), None, KINDS[kind], eps, stream)
-        return torch.from_numpy(loss).to(rho.device)
+        return loss
 
     @staticmethod
     def backward(ctx, grad_out):

@@ -41,12 +41,27 @@ def plain_small(generation=0, area=None):
 SW_THRESHOLD_512 = 0.03
 
 
-def _blocks_512(grad, steps=10, lr=3e-2, kind=0):
+def _blocks_512(grad, steps=10, lr=3e-2, kind=0, raar=16, hio=4, er=2):
     blocks = []
-    for _ in range(16):
+    for _ in range(raar):
         blocks.append([("RAAR", [25, 0.87])] + ([("GRAD", [steps, lr, kind])] if grad else []))
-    blocks += [[("HIO", [25, 0.95])]] * 4 + [[("ER", [25])]] * 2
+    blocks += [[("HIO", [25, 0.95])]] * hio + [[("ER", [25])]] * er
     return blocks
+
+
+def blocks_512(raar=16, hio=4, er=2, grad_steps=0, lr=3e-2, kind=0):
+    """A 512^2 schedule with `raar` RAAR, `hio` HIO and `er` ER blocks of 25
+    iterations (shrink-wrap after each block, sigma 3 -> 1.5), and with
+    grad_steps > 0 that many support-projected Adam steps after each RAAR block:
+    a callable (generation, area, threshold=None) -> sequence."""
+    def sched(generation=0, area=None, threshold=None):
+        blocks = _blocks_512(grad_steps > 0, grad_steps, lr, kind, raar, hio, er)
+        th = SW_THRESHOLD_512 if threshold is None else threshold
+        seq = []
+        for k, b in enumerate(blocks):
+            seq += b + [("SW", [_sigma(k, len(blocks)), th])]
+        return seq
+    return sched
 
 
 def plain_512(generation=0, area=None, threshold=None):

@@ -1,4 +1,6 @@
-"""End-to-end parity as a distribution (plain method, small workload).
+"""End-to-end parity as distributions over 32 seeds.
+
+Sequence only (test_plain_small_distribution_matches_oracle):
 
 32 seeded 128^2 polygon patterns (1e7 photons, 5 px beamstop), population 8
 from Solver.InitializePop, then the plain_small schedule (HIO/ER blocks of 20
@@ -9,6 +11,16 @@ individual's Fourier error and of its real-space error against the object
 (translation, global phase, scale and twin aligned): medians within 10 %
 (Fourier) and 0.05 (real space), and a two-sample Kolmogorov-Smirnov test not
 rejecting equality at the 1 % level.
+
+Whole memetic loop with one crossover generation (test_memetic_*): spring.MPR
+(spring/mpr.py:205-361: initialisation, evaluation, the generation-0 sequence,
+then shrink-wrap, alignment and averaging, Crossover, the sequence on parents
+and children, replacement of a parent by a better child) on the GPU against
+the FP32 oracle's restatement of the same loop (oracle/memetic.py), whose
+per-seed results are the committed fixtures tests/golden/memetic_*.json
+(written by tests/golden/make_memetic_fixtures.py). Same bars: medians of the
+final best error within 10 % and of the real-space error within 0.05,
+two-sample KS p > 0.01 for both.
 """
 import os
 
@@ -73,3 +85,47 @@ def test_plain_small_distribution_matches_oracle(orc):
     assert abs(np.median(g_rs) - np.median(c_rs)) <= 0.05
     assert ks_2samp(g_ef, c_ef).pvalue > 0.01
     assert ks_2samp(g_rs, c_rs).pvalue > 0.01
+
+
+def _fixture_cases():
+    import importlib.util
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "make_memetic_fixtures.py")
+    spec = importlib.util.spec_from_file_location("make_memetic_fixtures", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.mark.parametrize("n", [128])
+def test_memetic_generation_distribution_matches_oracle(n):
+    import json
+
+    from scipy.stats import ks_2samp
+
+    from s_spi_spring_amd import Pattern
+    from s_spi_spring_amd.mpr import MPR
+    mod = _fixture_cases()
+    c = mod.CASES[n]
+    fx = json.load(open(os.path.join(os.path.dirname(mod.__file__), c["file"])))
+    seeds = sorted(int(k) for k in fx["seeds"])
+    assert len(seeds) == 32
+    g_err, g_rs, c_err, c_rs = [], [], [], []
+    for seed in seeds:
+        I, truth = c["pattern"](seed)
+        m = MPR(Pattern.from_intensity(I), mod.settings_for(n, seed, truth))
+        m.run(log=False, schedule=c["schedule"])
+        best = min(m.reconstructions, key=lambda r: r.get_error())
+        g_err.append(best.get_error())
+        g_rs.append(metrics.real_space_error(best.get_density() * best.get_support(), truth))
+        ref = fx["seeds"][str(seed)]
+        c_err.append(ref["best_error"][-1])
+        c_rs.append(ref["real_space_error"])
+        print(f"n={n} seed {seed}: best error gpu {g_err[-1]:.4f} oracle {c_err[-1]:.4f}; real space gpu "
+              f"{g_rs[-1]:.4f} oracle {c_rs[-1]:.4f}", flush=True)
+    g_err, g_rs, c_err, c_rs = map(np.array, (g_err, g_rs, c_err, c_rs))
+    p_err, p_rs = ks_2samp(g_err, c_err).pvalue, ks_2samp(g_rs, c_rs).pvalue
+    print(f"n={n}: median best error gpu {np.median(g_err):.4f} oracle {np.median(c_err):.4f}; median real space "
+          f"gpu {np.median(g_rs):.4f} oracle {np.median(c_rs):.4f}; KS p {p_err:.3f} / {p_rs:.3f}")
+    assert abs(np.median(g_err) - np.median(c_err)) <= 0.10 * np.median(c_err)
+    assert abs(np.median(g_rs) - np.median(c_rs)) <= 0.05
+    assert p_err > 0.01 and p_rs > 0.01
