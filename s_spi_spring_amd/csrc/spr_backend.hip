@@ -223,7 +223,7 @@ template <int N, int CV> int launch_col_v(spr_ctx* c, int mode, float2* f, int c
     using G = typename ColVariant<N, CV>::G;
     if (G::TW_GLOBAL && !c->twtab[CV]) return fail(c, SPR_ERR_STATE, "twiddle table missing");
     const ColArgs a{f, c->amp, c->tw, c->amp_t, c->amp_of ? c->amp_of + first : nullptr, c->twtab[CV]};
-    const dim3 grid((unsigned)((size_t)cnt * (N / G::C))), block(G::COL_THREADS);
+    const dim3 grid((unsigned)((size_t)cnt * (N / G::COL_STRIP))), block(G::COL_THREADS);
     switch (mode) {
         case COL_PROJ: hipLaunchKernelGGL((k_col<N, COL_PROJ, CV>), grid, block, 0, s, a); break;
         case COL_FWD:

@@ -37,9 +37,13 @@ constexpr int pow2_floor(int x) { return x >= 2 ? 2 * pow2_floor(x / 2) : 1; }
 // pass's twiddle table from global memory (through the caches) instead of
 // building it in LDS. CSK > 0 lays the column tile out unpadded with one spare
 // tile row after every 2^CSK rows (row i of the tile at (i + i/2^CSK) * C; PAD
-// is then unused).
+// is then unused). CGR > 1 gives each column-pass workgroup CGR adjacent groups
+// of C columns (a strip of C*CGR columns, so that C*CGR = 16 reads whole
+// 128-byte row segments), transformed one group after another through the same
+// C-column LDS tile; WPE > 0 sets the column pass's waves per SIMD (its
+// register budget) instead of deriving it from the workgroups a CU holds.
 template <int N, class PL_ = RadixPlan<N>, int CCOLS = 0, int RTHREADS = 256, int RPF = -1, int PAD = 1, int TWG = 0,
-          int RSH = 3, int CSK = 0>
+          int RSH = 3, int CSK = 0, int CGR = 1, int WPE = 0>
 struct GeoT {
     using PL = PL_;
     static constexpr bool TW_GLOBAL = TWG != 0;
@@ -64,6 +68,8 @@ struct GeoT {
 #endif
     // columns per column-pass workgroup: a power of two (it divides N)
     static constexpr int C = CCOLS ? CCOLS : (pow2_floor(SPR_COL_COLUMNS / N) < N ? pow2_floor(SPR_COL_COLUMNS / N) : N);
+    static constexpr int COL_GROUPS = CGR;                // groups of C columns per column-pass workgroup
+    static constexpr int COL_STRIP = C * CGR;             // columns per column-pass workgroup
     static constexpr bool COL_WAVE = SPR_COL_WAVE && T <= 64;
     static constexpr int COL_THREADS = C * T;
     static constexpr int CS = C + PAD;                    // padded LDS row length of a column tile
@@ -91,9 +97,10 @@ struct GeoT {
     static constexpr int COL_WG_SLOTS = 32 / (COL_THREADS / 64);
     static constexpr int COL_WG = COL_WG_LDS < COL_WG_SLOTS ? (COL_WG_LDS < SPR_COL_WG_PER_CU ? COL_WG_LDS : SPR_COL_WG_PER_CU)
                                                             : (COL_WG_SLOTS < SPR_COL_WG_PER_CU ? COL_WG_SLOTS : SPR_COL_WG_PER_CU);
-    static constexpr int COL_WAVES_PER_EU = (COL_WG > 0 ? COL_WG : 1) * COL_THREADS / 256;
+    static constexpr int COL_WAVES_PER_EU = WPE > 0 ? WPE : (COL_WG > 0 ? COL_WG : 1) * COL_THREADS / 256;
     static_assert(RW >= 1 && ROW_THREADS % T == 0 && N % RW == 0, "row geometry");
-    static_assert(COL_THREADS <= 1024 && N % C == 0, "column geometry");
+    static_assert(COL_THREADS <= 1024 && N % COL_STRIP == 0, "column geometry");
+    static_assert(CGR == 1 || !COL_WAVE, "column groups need the tile layout");
 };
 template <int N> using Geo = GeoT<N>;
 
@@ -120,11 +127,11 @@ SPR_COL_VARIANT(384, 3, RadixPlan<384>, 4)
 SPR_COL_VARIANT(512, 1, RadixPlan<512>, 8)
 SPR_COL_VARIANT(512, 2, Plan<512, 16, 16, 4, 8>, 16)
 SPR_COL_VARIANT(512, 3, RadixPlan<512>, 4)
-SPR_COL_VARIANT(768, 1, RadixPlan<768>, 16)
+SPR_COL_VARIANT(768, 1, RadixPlan<768>, 4, 256, -1, 1, 0, 3, 0, 4, 2)   // 4 groups of 4 columns
 SPR_COL_VARIANT(768, 2, RadixPlan<768>, 4)
-SPR_COL_VARIANT(1024, 1, Plan<1024, 16, 16, 8, 8>, 16)
+SPR_COL_VARIANT(1024, 1, Plan<1024, 16, 16, 8, 8>, 8, 256, -1, 1, 1, 3, 0, 2, 4)   // 2 groups of 8
 SPR_COL_VARIANT(1024, 2, Plan<1024, 16, 16, 8, 8>, 8)
-SPR_COL_VARIANT(1024, 3, RadixPlan<1024>, 4)
+SPR_COL_VARIANT(1024, 3, Plan<1024, 16, 16, 8, 8>, 8, 256, -1, 0, 1, 3, 0, 2, 4)   // 2 groups of 8, unpadded
 SPR_COL_VARIANT(1024, 4, Plan<1024, 16, 16, 8, 8>, 8, 256, -1, 0, 0, 3, 4)
 // (plan, columns, -, -, padding, global twiddles): tiles under 80 KB, two workgroups per CU
 SPR_COL_VARIANT(384, 4, RadixPlan<384>, 16, 256, -1, 1, 1)
@@ -145,6 +152,12 @@ SPR_COL_VARIANT(2048, 2, Plan<2048, 16, 16, 8, 16>, 8, 256, -1, 0, 1)
 // 57.4 us; at 256^2, 384^2 and 2048^2 the skewed tiles were slower.
 SPR_COL_VARIANT(512, 6, RadixPlan<512>, 8, 256, -1, 0, 0, 3, 3)
 SPR_COL_VARIANT(768, 6, RadixPlan<768>, 4, 256, -1, 0, 0, 3, 2)
+// (plan, columns, -, -, padding, -, -, -, column groups, waves per SIMD): strips
+// of 16 columns (whole 128-byte row segments) transformed as groups of 8 or 4
+// columns through one tile, so that one group's transforms overlap the next
+// group's loads and the previous group's stores
+SPR_COL_VARIANT(512, 7, RadixPlan<512>, 8, 256, -1, 1, 0, 3, 0, 2, 0)
+SPR_COL_VARIANT(768, 7, RadixPlan<768>, 8, 256, -1, 1, 0, 3, 0, 2, 4)
 // row variants: plan, -, threads per workgroup, prefetch mode
 SPR_ROW_VARIANT(256, 1, RadixPlan<256>, 0, 512)
 SPR_ROW_VARIANT(384, 1, RadixPlan<384>, 0, 512)
@@ -504,16 +517,22 @@ __attribute__((amdgpu_waves_per_eu((ColVariant<N, CV>::G::COL_WAVES_PER_EU)))) v
     const float2* tw = twl;
     if constexpr (G::TW_GLOBAL) tw = a.twtab;
     else tw_fill_p<PL>(twl, a.tw, tid, G::COL_THREADS);
-    constexpr int TILES = N / C;
-    const int strip = col_strip<N, C>(blockIdx.x);
+    constexpr int CG = G::COL_GROUPS;
+    constexpr int TILES = N / G::COL_STRIP;
+    const int strip = col_strip<N, G::COL_STRIP>(blockIdx.x);
     const int p = strip / TILES;
-    const int x0 = (strip % TILES) * C;
+    const int x0 = (strip % TILES) * G::COL_STRIP;
     // Block-uniform bases with 32-bit per-thread offsets (SGPR base addressing).
     float2* __restrict__ f = a.f + (size_t)p * N * N + x0;
     const int off = t * N + c;
-    float2 v[V];
+    // Group g of the strip is columns [g*C, (g+1)*C); the loads of one row's
+    // groups are issued back to back, so a wavefront reads whole lines.
+    float2 vg[CG][V];
 #pragma unroll
-    for (int r = 0; r < V; ++r) v[r] = ld_stream(f + off + r * T * N);
+    for (int r = 0; r < V; ++r)
+#pragma unroll
+        for (int g = 0; g < CG; ++g) vg[g][r] = ld_stream(f + off + g * C + r * T * N);
+    float2(&v)[V] = vg[0];
     if constexpr (WV) {
         constexpr int CST = G::CST;
         const int cc = tid / T;   // column owned by this thread's wavefront
@@ -541,19 +560,36 @@ __attribute__((amdgpu_waves_per_eu((ColVariant<N, CV>::G::COL_WAVES_PER_EU)))) v
         const float* __restrict__ amp = a.amp + amp_offset<N>(a.amp_of, p) + x0;
         using Acc = std::conditional_t<(G::COL_SKEW > 0), ColAccSkew<C, (G::COL_SKEW > 0 ? G::COL_SKEW : 1)>, ColAcc<CS>>;
         const Acc acc{lds + c};
+        // The amplitude of group 0 is loaded with the field; that of a later
+        // group when its turn starts, so that its latency hides behind the
+        // group's forward transform while only one group's values are live.
         float am[V];
         if constexpr (MODE == COL_PROJ && SPR_COL_AMP_PREFETCH) {
 #pragma unroll
             for (int q = 0; q < V; ++q) am[q] = amp[off + q * T * N];
         }
         __syncthreads();
-        if constexpr (MODE == COL_PROJ || MODE == COL_FWD) fft_line_p<PL, +1>(v, acc, tw, t);
-        if constexpr (MODE == COL_PROJ) {
 #pragma unroll
-            for (int q = 0; q < V; ++q)
-                v[q] = project_modulus(v[q], SPR_COL_AMP_PREFETCH ? am[q] : amp[off + q * T * N]);
+        for (int g = 0; g < CG; ++g) {
+            if constexpr (MODE == COL_PROJ && SPR_COL_AMP_PREFETCH) {
+                if (g > 0) {
+#pragma unroll
+                    for (int q = 0; q < V; ++q) am[q] = amp[off + g * C + q * T * N];
+                }
+            }
+            if constexpr (MODE == COL_PROJ || MODE == COL_FWD) fft_line_p<PL, +1>(vg[g], acc, tw, t);
+            if constexpr (MODE == COL_PROJ) {
+#pragma unroll
+                for (int q = 0; q < V; ++q)
+                    vg[g][q] = project_modulus(vg[g][q], SPR_COL_AMP_PREFETCH ? am[q] : amp[off + g * C + q * T * N]);
+            }
+            if constexpr (MODE == COL_PROJ || MODE == COL_INV) fft_line_p<PL, -1>(vg[g], acc, tw, t);
+            if constexpr (CG > 1) {
+#pragma unroll
+                for (int q = 0; q < V; ++q) st_stream(f + off + g * C + q * T * N, vg[g][q]);
+            }
         }
-        if constexpr (MODE == COL_PROJ || MODE == COL_INV) fft_line_p<PL, -1>(v, acc, tw, t);
+        if constexpr (CG > 1) return;
     }
 #pragma unroll
     for (int q = 0; q < V; ++q) st_stream(f + off + q * T * N, v[q]);

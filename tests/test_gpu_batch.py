@@ -90,3 +90,31 @@ def test_batch_argument_errors(hip):
     with pytest.raises(hipcore.SpringHipError, match="negative iteration"):
         b.run([("HIO", [-1, 0.9])])
     b.close()
+
+
+def test_batch_of_all_config_sizes_against_the_fp64_oracle(hip, orc):
+    """A batch with every size of the 256-pattern workload (256^2, 384^2, 512^2,
+    768^2, 1024^2; two individuals each) against the FP64 oracle directly: one
+    RAAR iteration with its trailing projection per individual (relative L2
+    <= 1e-5) and the Fourier error of the result (relative <= 1e-5)."""
+    from conftest import rel_l2
+    sizes, pops = [256, 384, 512, 768, 1024], [2, 2, 2, 2, 2]
+    cs = [case(n, P, seed=40 + i) for i, (n, P) in enumerate(zip(sizes, pops))]
+    print("batch: sizes", sizes, "populations", pops, flush=True)
+    b = hipcore.Batch([c[0] for c in cs], pops)
+    for k, (amp, rho, supp) in enumerate(cs):
+        b.set_population(k, rho, supp)
+    b.run([("RAAR", [1, 0.87])], reality=0.0)
+    ev = b.evaluate(0.0)
+    for k, (amp, rho, supp) in enumerate(cs):
+        got = b.get_population(k, support=False)
+        amp64 = amp.astype(np.float64)
+        for i in range(pops[k]):
+            want = orc.iterate("RAAR", rho[i].astype(np.complex128), supp[i], amp64, 1, 0.87, 0.0)
+            e = rel_l2(got[i], want)
+            ef = orc.get_error(want, supp[i], amp64, 0.0)[0]
+            print(f"n={sizes[k]} individual {i}: RAAR step rel-L2 vs FP64 {e:.2e}; E_F {ev['error_fourier'][k][i]:.6f} "
+                  f"vs {ef:.6f}", flush=True)
+            assert e < 1e-5
+            assert abs(ev["error_fourier"][k][i] - ef) <= 1e-5 * ef
+    b.close()

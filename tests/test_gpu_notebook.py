@@ -27,7 +27,7 @@ Generation 0: within 30 % (errors, OS). Its OS is N^2 over the support of the
 best individual, which the generation-0 sequence ends with SWA onto the area of
 the initial support: the disk of diameter supportsize = 70, 3851 pixels, so
 OS = 17.02 for every seed here. The reference's current sources do the same
-(spring/mpr.py:238 passes that area; alg_manager.py:105-142 ends the sequence
+(spring/mpr.py:243 passes that area; alg_manager.py:105-142 ends the sequence
 with SWA onto it; src/solver.cpp:908-916 draws the disk), so the logged 13.37
 (4902 pixels, a disk of diameter about 79) comes from another version of the
 code than the one the notebook ships with; the generation-0 errors (6.5e-02

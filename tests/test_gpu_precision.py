@@ -3,7 +3,7 @@ kernel result against the FP64 oracle from the same state, with the error
 stated (printed) and bounded.
 
 1024^2 (1e10 photons) and 2048^2: one FT, one HIO / RAAR / ER iteration,
-shrink-wrap (mask bit-exact outside the rounding band), get_error metrics; and
+shrink-wrap (masks bit-exact), get_error metrics; and
 at 1024^2 with 1e8 photons (the low-count gradient case) the Poisson gradient
 against the FP64 closed form and central finite differences of the FP64 loss
 (step 1e-3, 64 seeded pixels). Bars as in the smaller-size tests: fields
@@ -67,7 +67,7 @@ def test_shrinkwrap_and_error_precision(hip, orc, n):
     band = np.abs(b - thr) <= 1e-5 * thr
     diff = got_S != want
     print(f"n={n} SW: {diff.sum()} differing pixels, {band.sum()} in the rounding band")
-    assert (diff & ~band).sum() == 0 and band.sum() <= 4
+    assert diff.sum() == 0
 
 
 def test_low_count_gradient_1024(hip):

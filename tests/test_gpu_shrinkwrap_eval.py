@@ -1,12 +1,13 @@
 """Parity of shrink-wrap (SW/SWA), fitness (get_error), symmetry and the op-list
 runner with the CPU oracle, through the C ABI.
 
-Supports are integer masks decided by FP32 comparisons on a blurred modulus; the
-GPU and the oracle compute that modulus with different (both FP32) FFTs, so the
-bar is: bit-exact on every pixel whose FP64-oracle margin to the threshold
-exceeds 1e-5 of the threshold, and the band inside that margin (reported) is a
-vanishing fraction of the pixels. Scalar metrics: relative error <= 1e-5 against
-the FP64 oracle.
+Supports are integer masks decided by FP32 comparisons on a blurred modulus.
+The bar is bit-exact masks on these fixed seeds. The GPU and the oracle compute
+that modulus with different FFTs, so a pixel whose blurred modulus lies within
+float rounding of the threshold (relative 1e-5, the reported "rounding band")
+could in principle be decided differently; on these inputs no pixel is (the
+counts are printed). Scalar metrics: relative error <= 1e-5 against the FP64
+oracle.
 """
 import numpy as np
 import pytest
@@ -26,8 +27,7 @@ def check_mask(got, want, b, thr, tag):
     diff = (got != want)
     bad = diff & ~ambiguous
     print(f"{tag}: {diff.sum()} differing pixels, {ambiguous.sum()} in the rounding band, {bad.sum()} outside it")
-    assert bad.sum() == 0
-    assert ambiguous.sum() <= 4
+    assert diff.sum() == 0
 
 
 @pytest.mark.parametrize("n,P", [(128, 6), (384, 2), (512, 3)])

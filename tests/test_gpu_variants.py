@@ -21,11 +21,11 @@ SIZES = [256, 384, 512, 768, 1024, 2048]
 # (size, kind, variant, variant with the same radix plan)
 SAME_PLAN = [(256, "col", 1, 0), (256, "col", 2, 0), (384, "col", 1, 0), (384, "col", 2, 0), (384, "col", 3, 0),
              (512, "col", 1, 0), (512, "col", 3, 0), (768, "col", 1, 0), (768, "col", 2, 0), (1024, "col", 2, 1),
-             (1024, "col", 3, 0), (1024, "col", 4, 1), (256, "row", 1, 0), (384, "row", 1, 0), (512, "row", 1, 0),
+             (1024, "col", 3, 1), (1024, "col", 4, 1), (256, "row", 1, 0), (384, "row", 1, 0), (512, "row", 1, 0),
              (768, "row", 1, 0), (768, "row", 2, 0), (1024, "row", 2, 1), (1024, "row", 3, 1),
              (384, "col", 4, 0), (512, "col", 4, 0), (512, "col", 5, 0), (768, "col", 3, 0), (768, "col", 4, 0),
              (768, "col", 5, 0), (1024, "col", 5, 1), (1024, "col", 6, 1), (1024, "col", 7, 0), (2048, "col", 1, 0),
-             (1024, "row", 4, 1), (2048, "row", 2, 1)]
+             (1024, "row", 4, 1), (2048, "row", 2, 1), (512, "col", 7, 0), (768, "col", 7, 0)]
 
 
 def pairs(n):
