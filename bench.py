@@ -38,7 +38,7 @@ FP32_PEAK_TFLOPS = 157.3
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_iteration_512_pop64_raar.json")
 COUNTER_CMD = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 90 rocprofv3 --pmc {ctr} --kernel-trace "
                "--output-format csv -d <dir> -o run -- python3 -u /root/repo/bench.py --steps 2 --warmup 1 "
-               "--no-cpu --calibrate {mib}")
+               "--no-cpu --prewarm 0 --ttr-seeds 0 --calibrate {mib}{extra}")
 
 
 def algorithmic_bytes(n, P, alg):
@@ -133,7 +133,7 @@ def _counter_values(run_dir, counter):
     return vals
 
 
-def summarize_traffic(fetch_dir, write_dir, mib, out_path, log):
+def summarize_traffic(fetch_dir, write_dir, mib, out_path, log, extra=""):
     """HBM bytes per dispatch of every spr:: kernel from a FETCH_SIZE and a
     WRITE_SIZE counter run of `bench.py --calibrate MIB`. Each counter is scaled
     so that it reads MIB MiB for the three MIB MiB device-to-device copies (the
@@ -157,7 +157,8 @@ def summarize_traffic(fetch_dir, write_dir, mib, out_path, log):
         kernels[k] = {"dispatches": len(fetch[k]), "fetch_kib": round(f_kib, 1), "write_kib": round(w_kib, 1),
                       "hbm_bytes": int(round((f_kib * ff + w_kib * fw) * 1024))}
     out = {"what": "HBM bytes per dispatch (FETCH_SIZE x fetch_factor + WRITE_SIZE x write_factor, KiB x 1024)",
-           "commands": [COUNTER_CMD.format(ctr="FETCH_SIZE", mib=mib), COUNTER_CMD.format(ctr="WRITE_SIZE", mib=mib)],
+           "commands": [COUNTER_CMD.format(ctr="FETCH_SIZE", mib=mib, extra=extra),
+                        COUNTER_CMD.format(ctr="WRITE_SIZE", mib=mib, extra=extra)],
            "device_source_hash": device_source_hash(),
            "calibration": {"copy_mib": mib, "fetch_factor": round(ff, 4), "write_factor": round(fw, 4)},
            "kernels": kernels}
@@ -386,6 +387,11 @@ def main():
     ap.add_argument("--plain-groups", default="", metavar="K,K",
                     help="time-to-rf: also run the plain schedule K individuals per launch for each K")
     ap.add_argument("--generations", type=int, default=10)
+    ap.add_argument("--prewarm", type=float, default=0.5, help="seconds of untimed iterations before the warmup")
+    ap.add_argument("--traffic-out", default=TRAFFIC_FILE,
+                    help="--summarize-traffic: output file (default: the headline workload's summary)")
+    ap.add_argument("--traffic-args", default="",
+                    help="--summarize-traffic: the workload arguments the counter runs used (recorded)")
     ap.add_argument("--ttr-seeds", type=int, default=32,
                     help="default run: seeds of the time-to-R_F fields (0 leaves them out)")
     ap.add_argument("--ttr-group-seeds", type=int, default=8,
@@ -410,7 +416,8 @@ def main():
         print(msg, file=sys.stderr, flush=True)
 
     if args.summarize_traffic:
-        summarize_traffic(*args.summarize_traffic, args.calibrate or 256, TRAFFIC_FILE, log)
+        summarize_traffic(*args.summarize_traffic, args.calibrate or 256, args.traffic_out, log,
+                          (" " + args.traffic_args) if args.traffic_args else "")
         return
 
     if args.gpus != 1:
@@ -448,7 +455,7 @@ def main():
     # Untimed pre-warm of about half a second, so that the clocks have ramped up
     # whatever --warmup is, then the requested warmup steps.
     t_pre = time.perf_counter()
-    while time.perf_counter() - t_pre < 0.5:
+    while time.perf_counter() - t_pre < args.prewarm:
         be.iterate_async(args.alg, args.block, args.beta, 0.0)
         be.sync()
     log("warmup")

@@ -21,16 +21,30 @@ each pinned in tests/test_oracle_*.py); run_memetic restates MPR.run.
 Context random streams: nthreads std::mt19937 streams seeded (i+1)*seed
 (src/solver.cpp:32-39); individual / child i draws from stream i % nthreads,
 one realisation of the reference's dynamic OpenMP schedule.
+
+Pins: every operation is one of the oracle's pinned restatements (or a numpy
+mean for the averages). The loop order (which operation follows which, the
+replacement rule) restates spring/mpr.py and is checked by reading only:
+parity unpinned for the order itself.
 """
 from __future__ import annotations
 
 import ctypes
+import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
 from . import oracle as orc
 
 F32 = np.float32
+_POOL = ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1)))
+
+
+def _each(fn, items):
+    """fn over items on host threads (the oracle's ctypes calls release the
+    GIL); results in order. Each call touches only its own item."""
+    return list(_POOL.map(fn, items))
 
 
 class Rec:
@@ -91,8 +105,8 @@ class OracleSolver:
 
     # --- metrics ------------------------------------------------------------
     def evaluate(self, recs):
-        for r in recs:
-            ef, er, cs, ms = orc.get_error(r.rho, r.S, self.amp, self.reality)
+        out = _each(lambda r: orc.get_error(r.rho, r.S, self.amp, self.reality), recs)
+        for r, (ef, er, cs, ms) in zip(recs, out):
             r.error_fourier, r.error_real = F32(ef), F32(er)
             reality = F32(1.0 - cs / ms)
             r.error = F32(r.error_fourier * (1.0 + self.realpen * reality))
@@ -115,11 +129,9 @@ class OracleSolver:
                 rho = orc.iterate(name, rho, S, self.amp, max(0, int(p[0])), beta, self.reality,
                                   nthreads=self.omp_threads)
             elif name == "SW":
-                for i in range(P):
-                    S[i] = orc.sw(rho[i], S[i], self.amp, p[0], p[1])
+                S = np.stack(_each(lambda i: orc.sw(rho[i], S[i], self.amp, p[0], p[1]), range(P)))
             elif name == "SWA":
-                for i in range(P):
-                    S[i] = orc.swa(rho[i], S[i], self.amp, p[0], int(p[1]))
+                S = np.stack(_each(lambda i: orc.swa(rho[i], S[i], self.amp, p[0], int(p[1])), range(P)))
             else:
                 raise ValueError(f"operation {name} has no oracle restatement")
         for i, r in enumerate(recs):
@@ -128,16 +140,19 @@ class OracleSolver:
         return recs
 
     def sw(self, recs, sigma, threshold):
-        for r in recs:
-            r.S = orc.sw(r.rho, r.S, self.amp, sigma, threshold)
+        for r, S in zip(recs, _each(lambda r: orc.sw(r.rho, r.S, self.amp, sigma, threshold), recs)):
+            r.S = S
         return recs
 
     def reshift(self, recs, ref):
         ref_rho = ref.rho.copy()
-        for r in recs:
+
+        def one(r):
             sx, sy, ph, flip = orc.get_shift(r.rho, ref_rho, self.amp, self.shiftmin, self.shiftmax,
                                              unit_scale_ft=True)
-            r.rho, r.S = orc.reshift(r.rho, r.S, sx, sy, ph, flip)
+            return orc.reshift(r.rho, r.S, sx, sy, ph, flip)
+        for r, (rho, S) in zip(recs, _each(one, recs)):
+            r.rho, r.S = rho, S
         return recs
 
     def _navg(self, P, fraction):
@@ -155,8 +170,8 @@ class OracleSolver:
     def get_average_module(self, recs, ft, fraction):
         idx = self.sorted_by_error(recs)[:self._navg(len(recs), fraction)]
         tot = np.zeros((self.n, self.n), np.float64)
-        for i in idx:
-            tot += np.abs(orc.ft(recs[i].rho)) if ft else np.abs(recs[i].rho)
+        for m in _each(lambda i: np.abs(orc.ft(recs[i].rho)) if ft else np.abs(recs[i].rho), idx):
+            tot += m
         return (tot / len(idx)).astype(np.float32)
 
     def crossover(self, recs, best, avg, crossbalance, crossdiff, crossexp, crossft, crosssym, use_average,
@@ -176,7 +191,8 @@ class OracleSolver:
         fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_float, ctypes.c_int, ctypes.c_float,
                        ctypes.c_float, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                        ctypes.c_void_p, ctypes.c_int]
-        children = []
+        # the draws in child order (each stream serves its children in turn), then the children
+        jobs = []
         for i in range(P):
             rc = fn(self.streams[i % len(self.streams)].h, P, orc._p(order), float(suppdim), n, float(crossexp),
                     float(crossft), float(crossbalance), orc._p(di), orc._p(df), orc._p(cm), orc._p(cl), cap)
@@ -187,9 +203,15 @@ class OracleSolver:
                         squaresizey=int(di[6]), offsetx=int(di[7]), offsety=int(di[8]), crossft=int(di[9]),
                         crosssym=int(bool(crosssym)), rotangle=float(df[0]), crossdiff=float(crossdiff),
                         use_average=float(use_average))
+            jobs.append((i, (p1, p2, p3), desc, cm[:int(cl[0])].copy()))
+
+        def child(job):
+            i, (p1, p2, p3), desc, cmap = job
             parents = [(recs[j].rho, recs[j].S) for j in (i, p1, p2, p3)]
-            rho, S = orc.cross_child(self.amp, avg_mod, parents, avg.rho, cm[:int(cl[0])].copy(), desc,
-                                     self.shiftmin, self.shiftmax, self.reality, True)
+            return orc.cross_child(self.amp, avg_mod, parents, avg.rho, cmap, desc, self.shiftmin, self.shiftmax,
+                                   self.reality, True)
+        children = []
+        for i, (rho, S) in enumerate(_each(child, jobs)):
             son = recs[i].copy()
             son.rho = rho
             if update_support:

@@ -49,17 +49,21 @@ def _blocks_512(grad, steps=10, lr=3e-2, kind=0, raar=16, hio=4, er=2):
     return blocks
 
 
-def blocks_512(raar=16, hio=4, er=2, grad_steps=0, lr=3e-2, kind=0):
+def blocks_512(raar=16, hio=4, er=2, grad_steps=0, lr=3e-2, kind=0, polish=None):
     """A 512^2 schedule with `raar` RAAR, `hio` HIO and `er` ER blocks of 25
-    iterations (shrink-wrap after each block, sigma 3 -> 1.5), and with
-    grad_steps > 0 that many support-projected Adam steps after each RAAR block:
-    a callable (generation, area, threshold=None) -> sequence."""
+    iterations (shrink-wrap after each block, sigma 3 -> 1.5), with
+    grad_steps > 0 that many support-projected Adam steps after each RAAR block,
+    and with polish = (steps, lr, kind) support-projected Adam steps after the
+    last shrink-wrap (on the support the generation ends with): a callable
+    (generation, area, threshold=None) -> sequence."""
     def sched(generation=0, area=None, threshold=None):
         blocks = _blocks_512(grad_steps > 0, grad_steps, lr, kind, raar, hio, er)
         th = SW_THRESHOLD_512 if threshold is None else threshold
         seq = []
         for k, b in enumerate(blocks):
             seq += b + [("SW", [_sigma(k, len(blocks)), th])]
+        if polish:
+            seq.append(("GRAD", [polish[0], polish[1], polish[2]]))
         return seq
     return sched
 
