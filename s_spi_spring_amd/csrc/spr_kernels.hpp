@@ -525,13 +525,15 @@ __attribute__((amdgpu_waves_per_eu((ColVariant<N, CV>::G::COL_WAVES_PER_EU)))) v
     // Block-uniform bases with 32-bit per-thread offsets (SGPR base addressing).
     float2* __restrict__ f = a.f + (size_t)p * N * N + x0;
     const int off = t * N + c;
-    // Group g of the strip is columns [g*C, (g+1)*C); the loads of one row's
-    // groups are issued back to back, so a wavefront reads whole lines.
+    // Group g of the strip is columns [g*C, (g+1)*C). The loads are issued
+    // group after group, so that the wait for group 0's values leaves the later
+    // groups' loads in flight during group 0's transforms (the other half of
+    // each line comes from the L2).
     float2 vg[CG][V];
 #pragma unroll
-    for (int r = 0; r < V; ++r)
+    for (int g = 0; g < CG; ++g)
 #pragma unroll
-        for (int g = 0; g < CG; ++g) vg[g][r] = ld_stream(f + off + g * C + r * T * N);
+        for (int r = 0; r < V; ++r) vg[g][r] = ld_stream(f + off + g * C + r * T * N);
     float2(&v)[V] = vg[0];
     if constexpr (WV) {
         constexpr int CST = G::CST;

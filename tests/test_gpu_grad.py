@@ -116,6 +116,38 @@ def test_support_refinement_matches_torch_adam(hip):
     assert not np.array_equal(got[0][supp[0] == 1], rho[0][supp[0] == 1])
 
 
+def test_refinement_clears_its_moments_beyond_2g_elements(hip):
+    """The Adam moments of a refinement are cleared at every call, also when the
+    moment arrays hold more than 2^31 32-bit words (1024^2 with 1100 fields:
+    2 N^2 P = 2.3e9). Three individuals replicated on the device over 1100
+    slots are refined twice from the same start (the first call leaves
+    non-zero moments behind); the second call's results equal those of a
+    fresh three-field handle bit for bit (the per-field arithmetic does not
+    depend on the population size)."""
+    n, P = 1024, 1100
+    amp, rho, supp = case(n, 3, seed=41)
+    print(f"refinement of {P} fields of {n}^2 (moments {2 * n * n * P} words)", flush=True)
+    big = hip.Backend(amp, capacity=P + 3)
+    big.put_individuals([P, P + 1, P + 2], list(rho), list(supp))
+    src = [P + (i % 3) for i in range(P)] + [-1, -1, -1]
+    big.gather(src)
+    big.set_active(P)
+    big.refine_support(3, lr_rel=2e-2)
+    big.gather(src)
+    big.set_active(P)
+    big.refine_support(3, lr_rel=2e-2)
+    got = big.fetch_individuals([0, 1, 2, P - 1], support=False)
+    big.close()
+    small = hip.Backend(amp, capacity=3)
+    small.set_population(rho, supp)
+    small.refine_support(3, lr_rel=2e-2)
+    want = small.get_population(support=False)
+    small.close()
+    for i in range(3):
+        assert np.array_equal(got[i], want[i]), i
+    assert np.array_equal(got[3], want[(P - 1) % 3])
+
+
 def test_low_count_refinement_lowers_the_poisson_loss(hip):
     """The differentiation workload's step at its size: 1024^2 truncated
     octahedron with 1e8 photons, after a RAAR block, 20 support-projected Adam
