@@ -31,6 +31,8 @@ class FourierLoss(torch.autograd.Function):
     def forward(ctx, rho, backend, kind="poisson", eps=1e-3):
         if not rho.is_cuda or rho.dtype != torch.complex64:
             raise TypeError("rho must be a complex64 tensor on the GPU")
+        if rho.device.index != backend.device:
+            raise ValueError(f"rho is on cuda:{rho.device.index}, the backend's amplitude on cuda:{backend.device}")
         rho = rho.contiguous()
         if rho.dim() == 2:
             rho = rho.unsqueeze(0)

@@ -203,6 +203,7 @@ class Backend:
             raise SpringHipError(self.lib.spr_last_error(None).decode())
         self.h = h
         self.n = n
+        self.device = int(device)
         self.capacity = int(capacity)
         self.P = 0
 
