@@ -41,23 +41,26 @@ def plain_small(generation=0, area=None):
 SW_THRESHOLD_512 = 0.03
 
 
-def _blocks_512(grad, steps=10, lr=3e-2, kind=0, raar=16, hio=4, er=2):
+def _blocks_512(grad, steps=10, lr=3e-2, kind=0, raar=16, hio=4, er=2, grad_all=False):
+    g = [("GRAD", [steps, lr, kind])] if grad else []
     blocks = []
     for _ in range(raar):
-        blocks.append([("RAAR", [25, 0.87])] + ([("GRAD", [steps, lr, kind])] if grad else []))
-    blocks += [[("HIO", [25, 0.95])]] * hio + [[("ER", [25])]] * er
+        blocks.append([("RAAR", [25, 0.87])] + g)
+    blocks += [[("HIO", [25, 0.95])] + (g if grad_all else [])] * hio
+    blocks += [[("ER", [25])] + (g if grad_all else [])] * er
     return blocks
 
 
-def blocks_512(raar=16, hio=4, er=2, grad_steps=0, lr=3e-2, kind=0, polish=None):
+def blocks_512(raar=16, hio=4, er=2, grad_steps=0, lr=3e-2, kind=0, polish=None, grad_all=False):
     """A 512^2 schedule with `raar` RAAR, `hio` HIO and `er` ER blocks of 25
     iterations (shrink-wrap after each block, sigma 3 -> 1.5), with
     grad_steps > 0 that many support-projected Adam steps after each RAAR block,
-    and with polish = (steps, lr, kind) support-projected Adam steps after the
-    last shrink-wrap (on the support the generation ends with): a callable
-    (generation, area, threshold=None) -> sequence."""
+    (grad_all: after every block), and with polish = (steps, lr, kind)
+    support-projected Adam steps after the last shrink-wrap (on the support the
+    generation ends with): a callable (generation, area, threshold=None) ->
+    sequence."""
     def sched(generation=0, area=None, threshold=None):
-        blocks = _blocks_512(grad_steps > 0, grad_steps, lr, kind, raar, hio, er)
+        blocks = _blocks_512(grad_steps > 0, grad_steps, lr, kind, raar, hio, er, grad_all)
         th = SW_THRESHOLD_512 if threshold is None else threshold
         seq = []
         for k, b in enumerate(blocks):
