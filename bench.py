@@ -367,8 +367,8 @@ def batch_bench(args, log):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--pop", type=int, default=64)
     ap.add_argument("--block", type=int, default=25, help="iterations per step")
