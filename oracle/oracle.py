@@ -111,6 +111,42 @@ def iterate(alg, rho, S, amp, niter, beta=0.9, reality=0.0, nthreads=0):
     return r
 
 
+def fft2_fast(f, sign):
+    """The throughput port's unnormalised 2-D DFT (fast_port.hpp), FP32."""
+    out = _cfield(f, np.complex64).copy()
+    if lib().orc_fft2_fast(ctypes.c_int(out.shape[-1]), _p(out), ctypes.c_int(sign)) != 0:
+        raise ValueError("fast port: lengths 2^a 3^b only")
+    return out
+
+
+def apply_support_fast(alg, rho, S, buf=None, beta=0.9, reality=0.0):
+    """The throughput port's real-space update, FP32; returns (rho, buf)."""
+    r = _cfield(rho, np.complex64).copy()
+    b = _cfield(buf if buf is not None else rho, np.complex64).copy()
+    s = np.ascontiguousarray(S, dtype=np.int32)
+    fn = lib().orc_apply_support_fast
+    fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                   ctypes.c_float, ctypes.c_float]
+    fn(r.shape[-1], ALG[alg], _p(r), _p(s), _p(b), beta, reality)
+    return r, b
+
+
+def iterate_fast(alg, rho, S, amp, niter, beta=0.9, reality=0.0, nthreads=0):
+    """The throughput port of iterate() in FP32 (fast_port.hpp), for the CPU
+    baseline: same driver and per-pixel operators, a vectorised transform."""
+    r = _cfield(rho, np.complex64).copy()
+    n = r.shape[-1]
+    P = 1 if r.ndim == 2 else r.shape[0]
+    s = np.ascontiguousarray(np.broadcast_to(S, r.shape), dtype=np.int32)
+    a = np.ascontiguousarray(amp, dtype=np.float32)
+    fn = lib().orc_iterate_fast
+    fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                   ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int]
+    if fn(n, P, _p(r), _p(s), _p(a), ALG[alg], int(niter), beta, reality, int(nthreads)) != 0:
+        raise ValueError("fast port: lengths 2^a 3^b only")
+    return r
+
+
 def blur(rho, amp, sigma):
     """Shrink-wrap blurred modulus field b (complex) of SW/SWA."""
     rt, sfx, ct = _rt(rho.dtype.type)

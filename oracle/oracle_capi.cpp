@@ -6,6 +6,9 @@
 // Suffix _f32 runs the FP32 oracle, _f64 the FP64 oracle (inputs/outputs in that type).
 #include <omp.h>
 
+#include <memory>
+
+#include "fast_port.hpp"
 #include "spring_oracle.hpp"
 
 using namespace orc;
@@ -257,5 +260,42 @@ extern "C" int orc_crossover_draws_rng(void* g, int P, const int* sorted_idx, fl
     *cm_len = (int)cm.size();
     if (cm.size() > (size_t)cm_cap) return -1;
     std::copy(cm.begin(), cm.end(), cm_out);
+    return 0;
+}
+
+// Throughput port of the FP32 driver (fast_port.hpp), for the CPU baseline.
+extern "C" int orc_fft2_fast(int n, float* f, int sign) {
+    try {
+        fastport::LinePlan pl(n);
+        fastport::Work w(n);
+        fastport::fft2(pl, C(f), sign, w);
+    } catch (const std::exception&) {
+        return -1;
+    }
+    return 0;
+}
+extern "C" int orc_apply_support_fast(int n, int alg, float* rho, const int32_t* S, float* buf, float beta,
+                                      float reality) {
+    std::vector<uint8_t> undecided((size_t)n * n);
+    fastport::update(alg, C(rho), S, alg == ALG_ER ? nullptr : C(buf), beta, (size_t)n * n, reality,
+                     undecided.data());
+    return 0;
+}
+extern "C" int orc_iterate_fast(int n, int P, float* rho, const int32_t* S, const float* amp, int alg, int niter,
+                                float beta, float reality, int nthreads) {
+    std::unique_ptr<fastport::LinePlan> pl;
+    try {
+        pl.reset(new fastport::LinePlan(n));
+    } catch (const std::exception&) {
+        return -1;
+    }
+    const size_t np = (size_t)n * n;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads)
+    {
+        fastport::Work w(n);
+#pragma omp for schedule(dynamic)
+        for (int i = 0; i < P; ++i) fastport::iterate(*pl, C(rho) + i * np, S + i * np, amp, alg, niter, beta, reality, w);
+    }
     return 0;
 }
