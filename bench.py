@@ -44,10 +44,11 @@ COUNTER_CMD = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 90 rocprofv3 --
 def algorithmic_bytes(n, P, alg):
     """HBM bytes per launch of the two iteration kernels (algorithmic, no re-reads):
     column pass 16 B/px (+ the amplitude, 4 B/px once per launch); row update pass
-    33 B/px for HIO/RAAR, 17 B/px for ER."""
+    32.125 B/px for HIO/RAAR, 16.125 B/px for ER (the field read and written, the
+    state rho_n read and written for HIO/RAAR, the support as one bit per pixel)."""
     npx = n * n
     col = 16 * npx * P + 4 * npx
-    row = (33 if alg in ("HIO", "RAAR") else 17) * npx * P
+    row = (32.125 if alg in ("HIO", "RAAR") else 16.125) * npx * P
     return col, row
 
 
@@ -343,9 +344,10 @@ def batch_bench(args, log):
     dt = time.perf_counter() - t0
     ind_iters = args.steps * args.batch_block * b.total
     value = ind_iters / dt
-    # algorithmic bytes of one iteration of the whole batch: 49 B/px per individual
-    # (16 column + 33 row) and each pattern's amplitude once per column pass
-    it_bytes = sum(49.0 * n * n * P + 4.0 * n * n for n in sizes)
+    # algorithmic bytes of one iteration of the whole batch: 48.125 B/px per
+    # individual (16 column + 32.125 row) and each pattern's amplitude once per
+    # column pass
+    it_bytes = sum(48.125 * n * n * P + 4.0 * n * n for n in sizes)
     frac = it_bytes * (value / b.total) / 1e9 / HBM_PEAK_GBS
     log(f"{value:.0f} individual-iterations/s; whole iteration {it_bytes * value / b.total / 1e9:.0f} GB/s "
         f"algorithmic ({frac:.3f} of HBM peak)")
@@ -377,7 +379,7 @@ def main():
     ap.add_argument("--block", type=int, default=25, help="iterations per step")
     ap.add_argument("--alg", default="RAAR", choices=["RAAR", "HIO", "ER"])
     ap.add_argument("--beta", type=float, default=0.87)
-    ap.add_argument("--cpu-iters", type=int, default=50)
+    ap.add_argument("--cpu-iters", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true",
                     help="time the steps by wall clock only (no per-launch events, no roofline entry)")

@@ -55,6 +55,7 @@ struct spr_ctx {
     float2* work = nullptr;
     float2* work2 = nullptr;      // second half-transformed field per individual (correlations, |b|)
     uint8_t* supp = nullptr;
+    uint32_t* sbits = nullptr;    // supports one bit per pixel, packed at the start of every iteration block
     float* amp = nullptr;         // amplitudes [npat][n][n]
     float* amp_t = nullptr;       // amp transposed (read along columns by the column pass)
     int npat = 1;                 // patterns sharing this handle (one amplitude each)
@@ -333,6 +334,7 @@ RowArgs row_args(spr_ctx* c) {
     a.rho = c->rho;
     a.work = c->work;
     a.supp = c->supp;
+    a.sbits = c->sbits;
     a.tw = c->tw;
     a.beta = 0.f;
     a.inv_nn = 1.0f / (float)((double)c->n * c->n);
@@ -577,6 +579,17 @@ float key_value(unsigned long long k) {
 }
 int key_index(unsigned long long k) { return k ? (int)(0xFFFFFFFFu - (unsigned)(k & 0xFFFFFFFFu)) : 0; }
 
+// supp -> sbits for individuals [0, cnt) from the pointers given (one bit per
+// pixel, read by the row update passes).
+int pack_support(spr_ctx* c, const uint8_t* supp, uint32_t* bits, int cnt, hipStream_t s) {
+    const size_t nwords = (size_t)cnt * c->n * c->n / 32;
+    if (nwords == 0) return SPR_OK;
+    const dim3 grid((unsigned)std::min<size_t>((nwords + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_pack_support, grid, dim3(256), 0, s, supp, bits, nwords);
+    SPR_HIP(c, hipGetLastError());
+    return SPR_OK;
+}
+
 // Pipelined form of the iteration over two halves of the population: lane 1
 // (individuals [0, h), the context's stream) and lane 2 ([h, P), a second
 // stream) issue the same pass sequence; lane 2 starts after lane 1's first row
@@ -590,7 +603,9 @@ int queue_iterate_lanes(spr_ctx* c, int mode, int niter, const RowArgs& a) {
     b.rho += off;
     b.work += off;
     b.supp += off;
+    b.sbits += off / 32;
     hipStream_t s1 = c->stream, s2 = c->lane2;
+    SPR_TRY(pack_support(c, c->supp, c->sbits, c->P, s1));
     SPR_TRY(row_n(c, ROW_FWD, a, h, s1));
     SPR_HIP(c, hipEventRecord(c->ev_fork, s1));
     SPR_HIP(c, hipStreamWaitEvent(s2, c->ev_fork, 0));
@@ -630,12 +645,14 @@ int chunk_individuals(const spr_ctx* c) {
 int queue_iterate_block(spr_ctx* c, int mode, int niter, const RowArgs& a0) {
     const int chunk = chunk_individuals(c);
     const size_t np = (size_t)c->n * c->n;
+    SPR_TRY(pack_support(c, a0.supp, const_cast<uint32_t*>(a0.sbits), c->P, c->stream));
     for (int first = 0; first < c->P; first += chunk) {
         const int cnt = std::min(chunk, c->P - first);
         RowArgs a = a0;
         a.rho += first * np;
         a.work += first * np;
         a.supp += first * np;
+        a.sbits += first * np / 32;
         SPR_TRY(row(c, ROW_FWD, a, cnt));
         for (int i = 0; i < niter; ++i) {
             SPR_TRY(col(c, COL_PROJ, a.work, cnt, first));
@@ -976,6 +993,7 @@ int autotune(spr_ctx* c, int reps, float* col_ms, float* row_ms) {
     const VariantPair keep{c->colvar, c->rowvar};
     c->rowvar = 0;
     SPR_TRY(row_n(c, ROW_FWD, ra, c->P, c->stream));   // realistic spectra in work
+    SPR_TRY(pack_support(c, c->supp, c->sbits, c->P, c->stream));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     SPR_HIP(c, hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -1186,6 +1204,7 @@ static int create_context(spr_ctx** out, int n, int capacity, int npat, const fl
     if ((e = hipMalloc(&c->work, np * capacity * sizeof(float2))) != hipSuccess)
         return cleanup_fail("hipMalloc work", e);
     if ((e = hipMalloc(&c->supp, np * capacity)) != hipSuccess) return cleanup_fail("hipMalloc supp", e);
+    if ((e = hipMalloc(&c->sbits, np / 8 * capacity)) != hipSuccess) return cleanup_fail("hipMalloc sbits", e);
     if ((e = hipMalloc(&c->amp, np * npat * sizeof(float))) != hipSuccess) return cleanup_fail("hipMalloc amp", e);
     if ((e = hipMalloc(&c->amp_t, np * npat * sizeof(float))) != hipSuccess) return cleanup_fail("hipMalloc amp_t", e);
     if ((e = hipMalloc(&c->tw, n * sizeof(float2))) != hipSuccess) return cleanup_fail("hipMalloc tw", e);
@@ -1259,6 +1278,7 @@ int spr_destroy(spr_ctx* c) {
     (void)hipFree(c->rho);
     (void)hipFree(c->work);
     (void)hipFree(c->supp);
+    (void)hipFree(c->sbits);
     (void)hipFree(c->amp);
     (void)hipFree(c->amp_t);
     (void)hipFree(c->amp_of);

@@ -5,6 +5,8 @@
 //   rho  [P][N][N] float2   real-space densities (the state between operations)
 //   work [P][N][N] float2   half-transformed fields (row-transformed / column-processed)
 //   supp [P][N][N] uint8    supports (0/1)
+//   sbits[P][N*N/32] uint32 the supports one bit per pixel (bit i&31 of word i>>5),
+//                           packed from supp at the start of every iteration block
 //   amp  [N][N]    float    corner-origin amplitude with the mask encoding of the reference
 //   tw   [N]       float2   exp(-2*pi*i*m/N)
 //
@@ -12,9 +14,10 @@
 // that does not fit in one CU's LDS):
 //   column pass  work <- colIFFT( P_M( colFFT(work) ) )                     16 B/px
 //   row pass     rho' = rowIFFT(work)/N^2 ; rho <- update(rho', rho, S) ;
-//                work <- rowFFT(rho)                                      33 B/px
-// so HIO/RAAR move 49 B/px per individual-iteration; ER, which needs no rho_n,
-// moves 33 B/px and writes rho only at the end.
+//                work <- rowFFT(rho)                                  32.125 B/px
+// so HIO/RAAR move 48.125 B/px per individual-iteration; ER, which needs no
+// rho_n, moves 32.125 B/px and writes rho only at the end. (The support packing
+// reads 1 and writes 0.125 B/px once per block.)
 #pragma once
 
 #include <cstdint>
@@ -359,12 +362,36 @@ enum RowMode {
 struct RowArgs {
     float2* rho;
     float2* work;
-    const uint8_t* supp;
+    const uint8_t* supp;     // supports, one byte per pixel (on-chip blocks)
+    const uint32_t* sbits;   // supports, one bit per pixel (row update passes)
     const float2* tw;
     float beta;
     float inv_nn;  // 1/N^2 (exact for power-of-two N; within 1 ulp of dividing by N^2 otherwise)
     PhaseCut pc;
 };
+
+// Support bit of pixel i of a field: bit i&31 of word i>>5 (N*N and the
+// row-block offsets are multiples of 32).
+__device__ __forceinline__ uint8_t support_bit(const uint32_t* __restrict__ bits, int i) {
+    return (uint8_t)((bits[i >> 5] >> (i & 31)) & 1u);
+}
+
+// supp (bytes) -> bits: word w holds pixels [32w, 32w+32), bit k = (supp[32w+k] != 0).
+// Each thread reads its 32 bytes as two 16-byte loads (supp 16-byte aligned).
+__global__ void k_pack_support(const uint8_t* __restrict__ supp, uint32_t* __restrict__ bits, size_t nwords) {
+    const size_t step = (size_t)gridDim.x * blockDim.x;
+    for (size_t w = (size_t)blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += step) {
+        const uint4 lo = *reinterpret_cast<const uint4*>(supp + 32 * w);
+        const uint4 hi = *reinterpret_cast<const uint4*>(supp + 32 * w + 16);
+        const unsigned x[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+        uint32_t m = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) m |= (uint32_t)(((x[k] >> (8 * j)) & 0xffu) != 0u) << (4 * k + j);
+        bits[w] = m;
+    }
+}
 
 // Loads of the state rho_n and the support issued with the field, ahead of the
 // first transform: 0 none, 1 rho_n, 2 rho_n and the support. At 512^2 the RAAR
@@ -391,7 +418,7 @@ __global__ __launch_bounds__((RowVariant<N, RV>::G::ROW_THREADS)) void k_row(Row
     const size_t blk = (size_t)blockIdx.x * G::RW * N;
     float2* __restrict__ rho = a.rho + blk;
     float2* __restrict__ work = a.work + blk;
-    const uint8_t* __restrict__ supp = a.supp + blk;
+    const uint32_t* __restrict__ sbits = a.sbits + blk / 32;
     const int off = rl * N + t;
     const RowAccT<G::ROW_PAD_SHIFT> acc{lds + rl * G::RS};
     float2 v[V];
@@ -419,7 +446,7 @@ __global__ __launch_bounds__((RowVariant<N, RV>::G::ROW_THREADS)) void k_row(Row
 #pragma unroll
         for (int r = 0; r < V; ++r) {
             v[r] = ld_stream(work + off + r * T);
-            if constexpr (PREFETCH >= 2) sn[r] = supp[off + r * T];
+            if constexpr (PREFETCH >= 2) sn[r] = support_bit(sbits, off + r * T);
             if constexpr (MODE != ROW_ER && PREFETCH >= 1) bn[r] = ld_stream(rho + off + r * T);
         }
         fft_line_p<PL, -1, WV>(v, acc, tw, t);
@@ -427,7 +454,7 @@ __global__ __launch_bounds__((RowVariant<N, RV>::G::ROW_THREADS)) void k_row(Row
         for (int q = 0; q < V; ++q) {
             const int i = off + q * T;
             const float2 d = make_float2(v[q].x * a.inv_nn, v[q].y * a.inv_nn);  // rho'
-            const uint8_t s = PREFETCH >= 2 ? sn[q] : supp[i];
+            const uint8_t s = PREFETCH >= 2 ? sn[q] : support_bit(sbits, i);
             float2 o;
             if constexpr (MODE == ROW_ER) {
                 const float ts = (s != 0 && !phase_fails(d.x, d.y, a.pc)) ? 1.f : 0.f;
