@@ -154,7 +154,7 @@ def summarize_traffic(fetch_dir, write_dir, mib, out_path, log, extra=""):
     ff, fw = factor(fetch), factor(write)
     kernels = {}
     for k in sorted(fetch):
-        if not k.startswith("void spr::") or k not in write:
+        if not k.startswith(("void spr::", "spr::")) or k not in write:
             continue
         f_kib = sum(fetch[k]) / len(fetch[k])
         w_kib = sum(write[k]) / len(write[k])
