@@ -306,6 +306,48 @@ def differentiation_study(seeds, generations, log):
     print(json.dumps(out), flush=True)
 
 
+def refine_bench(n, P, steps, log):
+    """Throughput of the Adam refinement steps (spr_refine, Poisson loss, no
+    support mask) on a synthetic population: the time of `steps` more steps
+    than a short run, per step, so that the set-up and the final loss cancel.
+    One step is the column loss pass (field read and written, 16 B/px, and the
+    amplitude once per launch) and the row Adam pass (field, density and both
+    moments read and written, 64 B/px): 80 B/px algorithmic. One JSON line."""
+    from s_spi_spring_amd import hipcore, synth
+    I, _ = synth.octahedron_pattern(seed=0, n=n, photons=1e8) if n >= 512 else synth.polygon_pattern(seed=0, n=n)
+    amp = synth.corner_amplitude(I)
+    supp = synth.disk_support(n, max(8, int(0.3 * n)))
+    scale = float(np.sqrt(np.maximum(I, 0).sum()) / supp.sum())
+    rho = synth.random_start(n, P, supp, seed=1, scale=scale)
+    supps = np.broadcast_to(supp, rho.shape).astype(np.int32)
+    be = hipcore.Backend(amp, capacity=P, device=0)
+    short = 5
+
+    def timed(k):
+        be.set_population(rho, supps)
+        be.sync()
+        t0 = time.perf_counter()
+        be.refine(k, lr=1e-2)
+        return time.perf_counter() - t0
+
+    timed(short)   # warm-up (code load, clocks)
+    reps = [(timed(short), timed(short + steps)) for _ in range(3)]
+    be.close()
+    per_step = float(np.median([(b - a) / steps for a, b in reps]))
+    step_bytes = (16 + 64) * n * n * P + 4 * n * n
+    gbs = step_bytes / per_step / 1e9
+    log(f"refinement: {steps} steps over {P} individuals at {n}^2: {per_step * 1e3:.3f} ms per step, "
+        f"{gbs:.0f} GB/s algorithmic ({gbs / HBM_PEAK_GBS:.3f} of HBM peak)")
+    out = {"metric": "Adam refinement steps/s (masked-Poisson loss gradient)", "value": round(1.0 / per_step, 1),
+           "unit": "steps/s", "higher_is_better": True, "n_gpus": 1, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"octahedron {n}^2 pattern, 1e8 photons, population {P}, spr_refine (Poisson)",
+                      "n": n, "population": P, "steps": steps},
+           "ms_per_step": round(per_step * 1e3, 4),
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_step": step_bytes}}
+    print(json.dumps(out), flush=True)
+
+
 def batch_bench(args, log):
     """Mixed-size batch: `args.batch` independent polygon patterns with sizes
     drawn from {256, 384, 512, 768, 1024} and `args.batch_pop` individuals each,
@@ -405,6 +447,8 @@ def main():
                     help="shrink-wrap threshold of the time-to-R_F schedules (default 0.03)")
     ap.add_argument("--differentiation", type=int, default=0, metavar="SEEDS",
                     help="run the low-count 1024^2 differentiation workload on SEEDS patterns (--generations)")
+    ap.add_argument("--refine-steps", type=int, default=0, metavar="K",
+                    help="time K Adam refinement steps at --n/--pop instead (e.g. --n 1024 --pop 32)")
     ap.add_argument("--batch", type=int, default=0, metavar="PATTERNS",
                     help="mixed-size batch workload of PATTERNS patterns (256 in the batching configuration)")
     ap.add_argument("--batch-pop", type=int, default=32)
@@ -440,6 +484,9 @@ def main():
         return
     if args.differentiation:
         differentiation_study(args.differentiation, args.generations, log)
+        return
+    if args.refine_steps:
+        refine_bench(args.n, args.pop, args.refine_steps, log)
         return
 
     from s_spi_spring_amd import hipcore, synth

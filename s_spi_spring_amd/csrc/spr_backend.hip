@@ -816,20 +816,93 @@ int correlate_and_pick(spr_ctx* c, const float2* spec, const int* d_index, const
 // ---------------------------------------------------------------------------
 // Masked Fourier loss and gradient (spr_grad.hpp)
 // ---------------------------------------------------------------------------
-template <int N> int launch_col_loss(spr_ctx* c, LossArgs a, int items) {
-    hipLaunchKernelGGL((k_col_loss<N>), dim3((unsigned)((size_t)items * (N / Geo<N>::C))), dim3(Geo<N>::COL_THREADS),
-                       0, c->stream, a);
+// Column loss pass in column variant CV (single-group LDS tiles only; other
+// variants use variant 0). want_loss: write one loss partial per workgroup.
+template <int N, int CV> constexpr bool loss_variant() {
+    if constexpr (ColVariant<N, CV>::valid) {
+        using G = typename ColVariant<N, CV>::G;
+        return G::COL_GROUPS == 1 && !G::COL_WAVE;
+    }
+    return false;
+}
+template <int N, int CV> int launch_col_loss_v(spr_ctx* c, LossArgs a, int items, bool want_loss) {
+    using G = typename ColVariant<N, CV>::G;
+    if (G::TW_GLOBAL && !c->twtab[CV]) return fail(c, SPR_ERR_STATE, "twiddle table missing");
+    a.twtab = c->twtab[CV];
+    const dim3 grid((unsigned)((size_t)items * (N / G::COL_STRIP))), block(G::COL_THREADS);
+    if (want_loss) hipLaunchKernelGGL((k_col_loss<N, CV, true>), grid, block, 0, c->stream, a);
+    else hipLaunchKernelGGL((k_col_loss<N, CV, false>), grid, block, 0, c->stream, a);
     SPR_HIP(c, hipGetLastError());
     return SPR_OK;
 }
-int col_loss(spr_ctx* c, const LossArgs& a, int items) { SPR_DISPATCH_N(c, launch_col_loss, c, a, items); }
+template <int N> int launch_col_loss(spr_ctx* c, LossArgs a, int items, int cv, bool want_loss) {
+    switch (cv) {
+        case 1:
+            if constexpr (loss_variant<N, 1>()) return launch_col_loss_v<N, 1>(c, a, items, want_loss);
+            break;
+        case 2:
+            if constexpr (loss_variant<N, 2>()) return launch_col_loss_v<N, 2>(c, a, items, want_loss);
+            break;
+        case 3:
+            if constexpr (loss_variant<N, 3>()) return launch_col_loss_v<N, 3>(c, a, items, want_loss);
+            break;
+        case 4:
+            if constexpr (loss_variant<N, 4>()) return launch_col_loss_v<N, 4>(c, a, items, want_loss);
+            break;
+        case 5:
+            if constexpr (loss_variant<N, 5>()) return launch_col_loss_v<N, 5>(c, a, items, want_loss);
+            break;
+        case 6:
+            if constexpr (loss_variant<N, 6>()) return launch_col_loss_v<N, 6>(c, a, items, want_loss);
+            break;
+        case 7:
+            if constexpr (loss_variant<N, 7>()) return launch_col_loss_v<N, 7>(c, a, items, want_loss);
+            break;
+    }
+    return launch_col_loss_v<N, 0>(c, a, items, want_loss);
+}
+// The loss and its partials in the default geometry (partials per field:
+// n / cols_per_wg(n)); the refinement steps' gradient passes in the handle's
+// column variant, without partials.
+int col_loss(spr_ctx* c, const LossArgs& a, int items) { SPR_DISPATCH_N(c, launch_col_loss, c, a, items, 0, true); }
+int col_loss_step(spr_ctx* c, const LossArgs& a, int items) {
+    SPR_DISPATCH_N(c, launch_col_loss, c, a, items, c->colvar, false);
+}
 
-template <int N> int launch_row_adam(spr_ctx* c, const AdamRowArgs& a, bool masked, int items) {
-    const dim3 grid((unsigned)((size_t)items * N / Geo<N>::RW)), block(Geo<N>::ROW_THREADS);
-    if (masked) hipLaunchKernelGGL((k_row_adam<N, true>), grid, block, 0, c->stream, a);
-    else hipLaunchKernelGGL((k_row_adam<N, false>), grid, block, 0, c->stream, a);
+// Row pass of a refinement step in the handle's row variant.
+template <int N, int RV> int launch_row_adam_v(spr_ctx* c, const AdamRowArgs& a, bool masked, int items) {
+    using G = typename RowVariant<N, RV>::G;
+    const dim3 grid((unsigned)((size_t)items * N / G::RW)), block(G::ROW_THREADS);
+    if (masked) hipLaunchKernelGGL((k_row_adam<N, RV, true>), grid, block, 0, c->stream, a);
+    else hipLaunchKernelGGL((k_row_adam<N, RV, false>), grid, block, 0, c->stream, a);
     SPR_HIP(c, hipGetLastError());
     return SPR_OK;
+}
+template <int N> int launch_row_adam(spr_ctx* c, const AdamRowArgs& a, bool masked, int items) {
+    switch (c->rowvar) {
+        case 1:
+            if constexpr (RowVariant<N, 1>::valid) return launch_row_adam_v<N, 1>(c, a, masked, items);
+            break;
+        case 2:
+            if constexpr (RowVariant<N, 2>::valid) return launch_row_adam_v<N, 2>(c, a, masked, items);
+            break;
+        case 3:
+            if constexpr (RowVariant<N, 3>::valid) return launch_row_adam_v<N, 3>(c, a, masked, items);
+            break;
+        case 4:
+            if constexpr (RowVariant<N, 4>::valid) return launch_row_adam_v<N, 4>(c, a, masked, items);
+            break;
+        case 5:
+            if constexpr (RowVariant<N, 5>::valid) return launch_row_adam_v<N, 5>(c, a, masked, items);
+            break;
+        case 6:
+            if constexpr (RowVariant<N, 6>::valid) return launch_row_adam_v<N, 6>(c, a, masked, items);
+            break;
+        case 7:
+            if constexpr (RowVariant<N, 7>::valid) return launch_row_adam_v<N, 7>(c, a, masked, items);
+            break;
+    }
+    return launch_row_adam_v<N, 0>(c, a, masked, items);
 }
 int row_adam(spr_ctx* c, const AdamRowArgs& a, bool masked, int items) {
     SPR_DISPATCH_N(c, launch_row_adam, c, a, masked, items);
@@ -849,8 +922,8 @@ int queue_adam_steps(spr_ctx* c, int steps, float lr, const float* lrp, int kind
     const float b1 = 0.9f, b2 = 0.999f;
     float p1 = 1.f, p2 = 1.f;
     for (int s = 1; s <= steps; ++s) {
-        LossArgs la{c->work, c->amp, c->amp_of, c->tw, c->partial, eps, kind};
-        SPR_TRY2(col_loss(c, la, P));
+        LossArgs la{c->work, c->amp, c->amp_of, c->tw, nullptr, eps, kind, nullptr};
+        SPR_TRY2(col_loss_step(c, la, P));
         p1 *= b1;
         p2 *= b2;
         AdamRowArgs ar{c->rho, (float2*)c->adam_m.p, (float2*)c->adam_v.p, c->work, c->supp, lrp, c->tw,
@@ -878,7 +951,7 @@ struct StreamSwap {
 int queue_loss_passes(spr_ctx* c, int P, const float2* rho, float2* field, float2* grad, int kind, float eps) {
     FieldBatch fwd{field, nullptr, c->tw, 1.f, rho};
     SPR_TRY2(fft_rows(c, +1, fwd, P));
-    LossArgs la{field, c->amp, c->amp_of, c->tw, c->partial, eps, kind};
+    LossArgs la{field, c->amp, c->amp_of, c->tw, c->partial, eps, kind, nullptr};
     SPR_TRY2(col_loss(c, la, P));
     if (!grad) return SPR_OK;
     FieldBatch adj{grad, nullptr, c->tw, 2.f, grad == field ? nullptr : field};

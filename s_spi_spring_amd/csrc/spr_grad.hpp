@@ -25,24 +25,35 @@ struct LossArgs {
     double* partial;     // one loss partial per workgroup
     float eps;
     int kind;
+    const float2* twtab; // the plan's twiddle table in global memory (column variants with TW_GLOBAL)
 };
 
-template <int N>
-__global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_loss(LossArgs a) {
-    using G = Geo<N>;
-    constexpr int V = G::V, T = G::T, C = G::C, NT = G::COL_THREADS, TILES = N / C;
-    __shared__ float2 tw[TwTable<N>::SIZE];
-    __shared__ float2 lds[N * G::CS];
-    __shared__ double red[4 * (NT / 64)];
+// Column pass of the loss in the geometry of column variant CV (the tile
+// layouts of the projection pass). WANT_LOSS: one loss partial per workgroup
+// into a.partial[strip] (summed in double); without it the pass computes only
+// the weighted spectrum (the refinement steps, whose losses are not read).
+template <int N, int CV = 0, bool WANT_LOSS = true>
+__global__ __launch_bounds__((ColVariant<N, CV>::G::COL_THREADS))
+__attribute__((amdgpu_waves_per_eu((ColVariant<N, CV>::G::COL_WAVES_PER_EU)))) void k_col_loss(LossArgs a) {
+    using G = typename ColVariant<N, CV>::G;
+    using PL = typename G::PL;
+    constexpr int V = G::V, T = G::T, C = G::C, NT = G::COL_THREADS, TILES = N / G::COL_STRIP;
+    static_assert(G::COL_GROUPS == 1 && !G::COL_WAVE, "loss passes use single-group LDS tiles");
+    __shared__ float2 twl[G::TW_GLOBAL ? 1 : TwTableP<PL>::SIZE];
+    __shared__ float2 lds[G::COL_TILE];
+    __shared__ double red[WANT_LOSS ? 4 * (NT / 64) : 1];
     const int tid = threadIdx.x, c = tid % C, t = tid / C;
-    tw_fill<N>(tw, a.tw, tid, NT);
-    const int strip = col_strip<N>(blockIdx.x);
+    const float2* tw = twl;
+    if constexpr (G::TW_GLOBAL) tw = a.twtab;
+    else tw_fill_p<PL>(twl, a.tw, tid, NT);
+    const int strip = col_strip<N, G::COL_STRIP>(blockIdx.x);
     const int item = strip / TILES;
     const int x0 = (strip % TILES) * C;
     float2* __restrict__ f = a.f + (size_t)item * N * N + x0;
     const float* __restrict__ amp = a.amp + amp_offset<N>(a.amp_of, item) + x0;
     const int off = t * N + c;
-    const ColAcc<G::CS> acc{lds + c};
+    using Acc = std::conditional_t<(G::COL_SKEW > 0), ColAccSkew<C, (G::COL_SKEW > 0 ? G::COL_SKEW : 1)>, ColAcc<G::CS>>;
+    const Acc acc{lds + c};
     float2 v[V];
     float Aq[V];
 #pragma unroll
@@ -51,7 +62,7 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_loss(LossArgs a) {
         Aq[r] = amp[off + r * T * N];
     }
     __syncthreads();
-    fft_line<N, +1>(v, acc, tw, t);
+    fft_line_p<PL, +1>(v, acc, tw, t);
     double l[1] = {0.0};
 #pragma unroll
     for (int q = 0; q < V; ++q) {
@@ -61,20 +72,22 @@ __global__ __launch_bounds__(Geo<N>::COL_THREADS) void k_col_loss(LossArgs a) {
         if (A >= 0.f) {
             if (a.kind == LOSS_POISSON) {
                 const float I = A * A;
-                l[0] += (double)u - (double)I * (double)logf(u + a.eps);
+                if constexpr (WANT_LOSS) l[0] += (double)u - (double)I * (double)logf(u + a.eps);
                 w = 1.f - I / (u + a.eps);
             } else {
                 const float m = sqrtf(u);
                 const float d = m - A;
-                l[0] += (double)d * (double)d;
+                if constexpr (WANT_LOSS) l[0] += (double)d * (double)d;
                 w = m > 0.f ? 1.f - A / m : 0.f;
             }
         }
         v[q] = make_float2(v[q].x * w, v[q].y * w);
     }
-    block_sum<1, NT>(l, red);
-    if (tid == 0) a.partial[strip] = l[0];
-    fft_line<N, -1>(v, acc, tw, t);
+    if constexpr (WANT_LOSS) {
+        block_sum<1, NT>(l, red);
+        if (tid == 0) a.partial[strip] = l[0];
+    }
+    fft_line_p<PL, -1>(v, acc, tw, t);
 #pragma unroll
     for (int q = 0; q < V; ++q) f[off + q * T * N] = v[q];
 }
@@ -132,17 +145,20 @@ struct AdamRowArgs {
     float lr, b1, b2, eps, bc1, bc2;
 };
 
-template <int N, bool MASKED>
-__global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row_adam(AdamRowArgs a) {
-    using G = Geo<N>;
+// In the geometry of row variant RV; the densities are loaded with the field,
+// before the adjoint transform, so that their latency hides behind it.
+template <int N, int RV, bool MASKED>
+__global__ __launch_bounds__((RowVariant<N, RV>::G::ROW_THREADS)) void k_row_adam(AdamRowArgs a) {
+    using G = typename RowVariant<N, RV>::G;
+    using PL = typename G::PL;
     constexpr int V = G::V, T = G::T;
     constexpr bool WV = T <= 64;
-    __shared__ float2 tw[TwTable<N>::SIZE];
+    __shared__ float2 tw[TwTableP<PL>::SIZE];
     __shared__ float2 lds[G::RW * G::RS];
     const int tid = threadIdx.x;
     const int t = tid % T;
     const int rl = tid / T;
-    tw_fill<N>(tw, a.tw, tid, G::ROW_THREADS);
+    tw_fill_p<PL>(tw, a.tw, tid, G::ROW_THREADS);
     const size_t blk = (size_t)blockIdx.x * G::RW * N;
     const int field = (int)(blk / ((size_t)N * N));
     const int off = rl * N + t;
@@ -150,18 +166,21 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row_adam(AdamRowArgs a)
     float2* __restrict__ m = a.m + blk;
     float2* __restrict__ v2 = a.v + blk;
     float2* __restrict__ work = a.work + blk;
-    const RowAcc acc{lds + rl * G::RS};
+    const RowAccT<G::ROW_PAD_SHIFT> acc{lds + rl * G::RS};
     const float lr = MASKED ? a.lrp[field] : a.lr;
-    float2 v[V];
+    float2 v[V], xs[V];
 #pragma unroll
-    for (int r = 0; r < V; ++r) v[r] = work[off + r * T];
+    for (int r = 0; r < V; ++r) {
+        v[r] = work[off + r * T];
+        xs[r] = x[off + r * T];
+    }
     __syncthreads();
-    fft_line<N, -1, WV>(v, acc, tw, t);
+    fft_line_p<PL, -1, WV>(v, acc, tw, t);
 #pragma unroll
     for (int q = 0; q < V; ++q) {
         const int i = off + q * T;
         const float2 gi = make_float2(v[q].x * 2.f, v[q].y * 2.f);
-        float2 xi = x[i];
+        float2 xi = xs[q];
         if (!MASKED || a.supp[blk + i]) {
             float2 mi = m[i], vi = v2[i];
             mi = make_float2(a.b1 * mi.x + (1.f - a.b1) * gi.x, a.b1 * mi.y + (1.f - a.b1) * gi.y);
@@ -174,7 +193,7 @@ __global__ __launch_bounds__(Geo<N>::ROW_THREADS) void k_row_adam(AdamRowArgs a)
         }
         v[q] = xi;
     }
-    fft_line<N, +1, WV>(v, acc, tw, t);
+    fft_line_p<PL, +1, WV>(v, acc, tw, t);
 #pragma unroll
     for (int q = 0; q < V; ++q) work[off + q * T] = v[q];
 }

@@ -85,10 +85,20 @@ def test_autograd_function(hip):
     assert fourier_loss(x.detach(), be, "poisson").grad_fn is None
 
 
-def test_device_adam_matches_torch_adam(hip):
+@pytest.mark.parametrize("n", [128, 512])
+def test_device_adam_matches_torch_adam(hip, n):
+    """spr_refine (its passes in the handle's tuned column and row variants:
+    variant 0 at 128^2, the default radix plan at 512^2) = torch.optim.Adam on
+    the gradient of the loss pass in the default geometry (FourierLoss). Where
+    the tuned variants use another radix plan (1024^2) twenty steps at this
+    step size amplify the plans' rounding differences (Adam's first steps move
+    every component by about lr whatever the gradient's size, so components
+    whose gradient is near zero take either sign), and the step is checked
+    component by component instead (test_refinement_step_follows_the_gradient)."""
     import torch
     from s_spi_spring_amd.grad import refine
-    amp, rho, supp = case(128, 2, seed=23)
+    print(f"refinement vs torch Adam at {n}^2", flush=True)
+    amp, rho, supp = case(n, 2, seed=23)
     be = hip.Backend(amp, capacity=2)
     ref, hist = refine(torch.from_numpy(rho).cuda(), be, steps=20, lr=1e-2)
     be.set_population(rho, supp)
@@ -98,13 +108,45 @@ def test_device_adam_matches_torch_adam(hip):
     assert np.all(loss < hist[0])   # the refinement lowers the loss
 
 
-def test_support_refinement_matches_torch_adam(hip):
+@pytest.mark.parametrize("kind", [0, 1])
+@pytest.mark.parametrize("n", [512, 1024])
+def test_refinement_step_follows_the_gradient(hip, n, kind):
+    """One spr_refine step in the handle's tuned variants. Adam's first step is
+    x -= lr * g / (|g| + 1e-8) per real component (m/(1-b1) = g, v/(1-b2) = g^2),
+    so every component whose FP64 gradient (np_loss_grad) is not negligible
+    moves by lr against its sign: this checks the gradient of the tuned column
+    loss pass and row Adam pass component by component."""
+    amp, rho, supp = case(n, 2, seed=25)
+    lr = 1e-3
+    be = hip.Backend(amp, capacity=2)
+    be.set_population(rho, supp)
+    print(f"one refinement step at {n}^2, loss kind {kind}, variants {be.variants()}", flush=True)
+    be.refine(1, lr=lr, kind=kind)
+    got = be.get_population(support=False)
+    be.close()
+    for i in range(2):
+        _, g = np_loss_grad(rho[i], amp, kind, 1e-3)
+        d = (got[i] - rho[i]).astype(np.complex128)
+        for comp, gc, dc in (("re", g.real, d.real), ("im", g.imag, d.imag)):
+            rms = np.sqrt(np.mean(gc ** 2))
+            sel = np.abs(gc) > max(1e-3 * rms, 1e-4)   # and |g| >> Adam's 1e-8
+            wrong = np.count_nonzero(np.sign(dc[sel]) != -np.sign(gc[sel]))
+            size = np.abs(np.abs(dc[sel]) - lr).max() / lr
+            print(f"  individual {i} {comp}: {sel.sum()} of {gc.size} components selected (rms {rms:.2e}), {wrong} moved the wrong way, "
+                  f"largest step-size error {size:.1e}")
+            assert wrong == 0
+            assert size < 1e-3
+
+
+@pytest.mark.parametrize("n", [128, 512])
+def test_support_refinement_matches_torch_adam(hip, n):
     """spr_refine_support (the GRAD operation) = torch.optim.Adam with one group
     per field, lr = lr_rel x RMS density on the support, gradient masked by it,
     from the densities set to 0 outside their supports."""
     import torch
     from s_spi_spring_amd.grad import refine_support
-    amp, rho, supp = case(128, 3, seed=24)
+    print(f"support refinement vs torch Adam at {n}^2", flush=True)
+    amp, rho, supp = case(n, 3, seed=24)
     supp[2] = 0                                     # an empty support: nothing moves
     be = hip.Backend(amp, capacity=3)
     ref = refine_support(torch.from_numpy(rho).cuda(), supp, be, steps=10, lr_rel=2e-2).cpu().numpy()
