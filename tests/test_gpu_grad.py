@@ -87,14 +87,13 @@ def test_autograd_function(hip):
 
 @pytest.mark.parametrize("n", [128, 512])
 def test_device_adam_matches_torch_adam(hip, n):
-    """spr_refine (its passes in the handle's tuned column and row variants:
-    variant 0 at 128^2, the default radix plan at 512^2) = torch.optim.Adam on
-    the gradient of the loss pass in the default geometry (FourierLoss). Where
-    the tuned variants use another radix plan (1024^2) twenty steps at this
-    step size amplify the plans' rounding differences (Adam's first steps move
-    every component by about lr whatever the gradient's size, so components
-    whose gradient is near zero take either sign), and the step is checked
-    component by component instead (test_refinement_step_follows_the_gradient)."""
+    """spr_refine = torch.optim.Adam on the gradient of the same loss pass
+    (FourierLoss). The two Adam updates order their operations differently, and
+    at 1024^2 twenty steps at this step size amplify that rounding beyond the
+    bar (Adam's first steps move every component by about lr whatever the
+    gradient's size, so a component whose gradient is near zero can take either
+    sign); there the step is checked component by component instead
+    (test_refinement_step_follows_the_gradient)."""
     import torch
     from s_spi_spring_amd.grad import refine
     print(f"refinement vs torch Adam at {n}^2", flush=True)
@@ -111,16 +110,16 @@ def test_device_adam_matches_torch_adam(hip, n):
 @pytest.mark.parametrize("kind", [0, 1])
 @pytest.mark.parametrize("n", [512, 1024])
 def test_refinement_step_follows_the_gradient(hip, n, kind):
-    """One spr_refine step in the handle's tuned variants. Adam's first step is
-    x -= lr * g / (|g| + 1e-8) per real component (m/(1-b1) = g, v/(1-b2) = g^2),
-    so every component whose FP64 gradient (np_loss_grad) is not negligible
-    moves by lr against its sign: this checks the gradient of the tuned column
-    loss pass and row Adam pass component by component."""
+    """One spr_refine step. Adam's first step is x -= lr * g / (|g| + 1e-8) per
+    real component (m/(1-b1) = g, v/(1-b2) = g^2), so every component whose
+    FP64 gradient (np_loss_grad) is not negligible moves by lr against its
+    sign: this checks the refinement's gradient pass (without loss partials)
+    and its Adam row pass component by component."""
     amp, rho, supp = case(n, 2, seed=25)
     lr = 1e-3
     be = hip.Backend(amp, capacity=2)
     be.set_population(rho, supp)
-    print(f"one refinement step at {n}^2, loss kind {kind}, variants {be.variants()}", flush=True)
+    print(f"one refinement step at {n}^2, loss kind {kind}", flush=True)
     be.refine(1, lr=lr, kind=kind)
     got = be.get_population(support=False)
     be.close()
