@@ -882,6 +882,28 @@ int queue_adam_steps(spr_ctx* c, int steps, float lr, const float* lrp, int kind
     return SPR_OK;
 }
 
+// The launches of spr_refine_support on the handle's stream, without waiting
+// for them (the GRAD operation of spr_run and spr_batch_run).
+int queue_refine_support(spr_ctx* c, int steps, float lr_rel, int kind, float eps) {
+    const int P = c->P;
+    if (P <= 0 || steps == 0) return SPR_OK;
+    const int np = c->n * c->n;
+    SPR_TRY(ensure(c, c->xphase, (size_t)P * sizeof(float)));
+    float* lrp = (float*)c->xphase.p;
+    // Start from the projection onto the supports: after an ER/HIO/RAAR block
+    // the densities are IFT(P_M(FT rho)), whose Fourier magnitudes already
+    // minimise the loss, so descending from them without the support
+    // constraint has nothing to gain.
+    const size_t total = (size_t)P * np;
+    hipLaunchKernelGGL(k_zero_outside, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->stream, c->rho,
+                       (const uint8_t*)c->supp, total);
+    SPR_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(k_support_scale, dim3(P), dim3(256), 0, c->stream, (const float2*)c->rho,
+                       (const uint8_t*)c->supp, np, lr_rel, lrp);
+    SPR_HIP(c, hipGetLastError());
+    return queue_adam_steps(c, steps, 0.f, lrp, kind, eps);
+}
+
 // Runs the handle's launches on another stream for the lifetime of the guard.
 struct StreamSwap {
     spr_ctx* c;
@@ -1641,9 +1663,7 @@ int spr_run(spr_ctx* c, const spr_op* ops, int nops, int nrep, float reality) {
                 case SPR_OP_RAAR: SPR_TRY(queue_iterate(c, SPR_RAAR, o.iterations, o.beta, reality)); break;
                 case SPR_OP_SW: SPR_TRY(queue_sw(c, o.sigma, o.threshold)); break;
                 case SPR_OP_SWA: SPR_TRY(queue_swa(c, o.sigma, o.pixels)); break;
-                case SPR_OP_GRAD:
-                    SPR_TRY(spr_refine_support(c, o.iterations, o.beta, o.pixels, o.threshold, nullptr));
-                    break;
+                case SPR_OP_GRAD: SPR_TRY(queue_refine_support(c, o.iterations, o.beta, o.pixels, o.threshold)); break;
             }
         }
     }
@@ -2030,24 +2050,9 @@ int spr_refine_support(spr_ctx* c, int steps, float lr_rel, int kind, float eps,
     if (!c) return fail(c, SPR_ERR_ARG, "NULL handle");
     if (steps < 0) return fail(c, SPR_ERR_ARG, "negative step count");
     if (kind != LOSS_POISSON && kind != LOSS_AMPLITUDE) return fail(c, SPR_ERR_ARG, "unknown loss kind");
-    const int P = c->P;
-    if (P <= 0 || steps == 0) return SPR_OK;
-    const int np = c->n * c->n;
+    if (c->P <= 0 || steps == 0) return SPR_OK;
     SPR_HIP(c, hipSetDevice(c->device));
-    SPR_TRY(ensure(c, c->xphase, (size_t)P * sizeof(float)));
-    float* lrp = (float*)c->xphase.p;
-    // Start from the projection onto the supports: after an ER/HIO/RAAR block
-    // the densities are IFT(P_M(FT rho)), whose Fourier magnitudes already
-    // minimise the loss, so descending from them without the support
-    // constraint has nothing to gain.
-    const size_t total = (size_t)P * np;
-    hipLaunchKernelGGL(k_zero_outside, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->stream, c->rho,
-                       (const uint8_t*)c->supp, total);
-    SPR_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(k_support_scale, dim3(P), dim3(256), 0, c->stream, (const float2*)c->rho,
-                       (const uint8_t*)c->supp, np, lr_rel, lrp);
-    SPR_HIP(c, hipGetLastError());
-    SPR_TRY(queue_adam_steps(c, steps, 0.f, lrp, kind, eps));
+    SPR_TRY(queue_refine_support(c, steps, lr_rel, kind, eps));
     if (loss) SPR_TRY(final_loss(c, kind, eps, loss));
     SPR_HIP(c, hipStreamSynchronize(c->stream));
     return harvest_timing(c);
@@ -2482,8 +2487,7 @@ int spr_batch_run(spr_batch* b, const spr_op* ops, int nops, int nrep, float rea
                     break;
                 case SPR_OP_GRAD:
                     for (auto& k : b->classes)
-                        SPR_TRY(bclass(b, k, spr_refine_support(k.ctx, o.iterations, o.beta, o.pixels, o.threshold,
-                                                                nullptr)));
+                        SPR_TRY(bclass(b, k, queue_refine_support(k.ctx, o.iterations, o.beta, o.pixels, o.threshold)));
                     break;
             }
         }
