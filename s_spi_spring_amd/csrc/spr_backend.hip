@@ -940,7 +940,7 @@ int queue_loss_grad(spr_ctx* c, int P, const float2* rho, float2* grad, int kind
 // Loss of every resident density after a refinement (host, P doubles).
 int final_loss(spr_ctx* c, int kind, float eps, double* loss) {
     const int P = c->P;
-    SPR_TRY2(queue_loss_grad(c, P, c->rho, c->work2, kind, eps));
+    SPR_TRY2(queue_loss_grad(c, P, c->rho, nullptr, kind, eps));
     std::vector<double> sm((size_t)P * SUMS_PER_ITEM);
     SPR_HIP(c, hipMemcpyAsync(sm.data(), c->sums, sm.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     SPR_HIP(c, hipStreamSynchronize(c->stream));

@@ -19,8 +19,8 @@ generation 0 and 1 and the real-space error of the final best individual
 (translation, global phase, scale and twin aligned, FP64).
 
 Usage: python tests/golden/make_memetic_fixtures.py {128,512} [first_seed last_seed]
-(the 512^2 file takes about 2 hours on 8 cores; seeds can be run in parts and
-are merged into the file).
+(the 512^2 file takes about 3 hours on 8 cores, 6 minutes per seed; seeds can
+be run in parts and are merged into the file).
 """
 import json
 import os
