@@ -78,7 +78,7 @@ struct spr_ctx {
         void* p = nullptr;
         size_t bytes = 0;
     };
-    Buf spec, xf, xs, supp2, fieldbuf, ampavg, xidx, shiftp, xchild, xmaps, xphase, adam_m, adam_v, gidx;
+    Buf spec, xf, xs, supp2, fieldbuf, ampavg, xidx, shiftp, xchild, xmaps, xphase, adam_m, adam_v, gidx, lrpart;
     Buf pinned;                   // pinned host staging of spr_put/fetch_individuals (hipHostMalloc)
     std::string err;
     // Opt-in per-launch timing: events bracket every launch on the stream.
@@ -828,14 +828,50 @@ template <int N, int CV> int launch_col_loss_v(spr_ctx* c, LossArgs a, int items
     SPR_HIP(c, hipGetLastError());
     return SPR_OK;
 }
+// Column variants whose tile holds the default radix plan compute the same
+// arithmetic as variant 0 (only the LDS layout and the columns per workgroup
+// differ).
+template <int N, int CV> constexpr bool same_plan_variant() {
+    if constexpr (ColVariant<N, CV>::valid) {
+        using G = typename ColVariant<N, CV>::G;
+        return G::COL_GROUPS == 1 && !G::COL_WAVE && std::is_same<typename G::PL, RadixPlan<N>>::value;
+    }
+    return false;
+}
 template <int N> int launch_col_loss(spr_ctx* c, LossArgs a, int items, bool want_loss) {
+    if (!want_loss) {
+        switch (c->colvar) {
+            case 1:
+                if constexpr (same_plan_variant<N, 1>()) return launch_col_loss_v<N, 1>(c, a, items, false);
+                break;
+            case 2:
+                if constexpr (same_plan_variant<N, 2>()) return launch_col_loss_v<N, 2>(c, a, items, false);
+                break;
+            case 3:
+                if constexpr (same_plan_variant<N, 3>()) return launch_col_loss_v<N, 3>(c, a, items, false);
+                break;
+            case 4:
+                if constexpr (same_plan_variant<N, 4>()) return launch_col_loss_v<N, 4>(c, a, items, false);
+                break;
+            case 5:
+                if constexpr (same_plan_variant<N, 5>()) return launch_col_loss_v<N, 5>(c, a, items, false);
+                break;
+            case 6:
+                if constexpr (same_plan_variant<N, 6>()) return launch_col_loss_v<N, 6>(c, a, items, false);
+                break;
+            case 7:
+                if constexpr (same_plan_variant<N, 7>()) return launch_col_loss_v<N, 7>(c, a, items, false);
+                break;
+        }
+    }
     return launch_col_loss_v<N, 0>(c, a, items, want_loss);
 }
 // The loss pass with one loss partial per workgroup (n / cols_per_wg(n) per
-// field), and the refinement steps' gradient pass without partials. Both run in
-// the default geometry: the tuned column variants measured no faster for this
-// pass (1024^2, 32 fields: 0.577 against 0.575 ms per refinement step), and one
-// geometry keeps the refinement's gradients those of spr_loss_grad.
+// field) in the default geometry, and the refinement steps' gradient pass
+// without partials, in the handle's column variant when that variant holds the
+// default radix plan (the same gradients as spr_loss_grad, bit for bit), else
+// in the default geometry too (at 1024^2 the radix-16 variants measured no
+// faster for this pass: 0.577 against 0.575 ms per refinement step).
 int col_loss(spr_ctx* c, const LossArgs& a, int items) { SPR_DISPATCH_N(c, launch_col_loss, c, a, items, true); }
 int col_loss_step(spr_ctx* c, const LossArgs& a, int items) { SPR_DISPATCH_N(c, launch_col_loss, c, a, items, false); }
 
@@ -898,8 +934,13 @@ int queue_refine_support(spr_ctx* c, int steps, float lr_rel, int kind, float ep
     hipLaunchKernelGGL(k_zero_outside, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, c->stream, c->rho,
                        (const uint8_t*)c->supp, total);
     SPR_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(k_support_scale, dim3(P), dim3(256), 0, c->stream, (const float2*)c->rho,
-                       (const uint8_t*)c->supp, np, lr_rel, lrp);
+    // Every supported n is a multiple of 8, so SUPPORT_CHUNKS = 64 divides n^2.
+    SPR_TRY(ensure(c, c->lrpart, (size_t)P * SUPPORT_CHUNKS * 2 * sizeof(double)));
+    hipLaunchKernelGGL(k_support_partials, dim3((unsigned)(P * SUPPORT_CHUNKS)), dim3(256), 0, c->stream,
+                       (const float2*)c->rho, (const uint8_t*)c->supp, np, (double*)c->lrpart.p);
+    SPR_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(k_support_lr, dim3((unsigned)((P + 63) / 64)), dim3(64), 0, c->stream, (const double*)c->lrpart.p,
+                       P, lr_rel, lrp);
     SPR_HIP(c, hipGetLastError());
     return queue_adam_steps(c, steps, 0.f, lrp, kind, eps);
 }
@@ -1336,7 +1377,7 @@ int spr_destroy(spr_ctx* c) {
     (void)hipFree(c->radix);
     (void)hipFree(c->hist);
     for (spr_ctx::Buf* b : {&c->spec, &c->xf, &c->xs, &c->supp2, &c->fieldbuf, &c->ampavg, &c->xidx, &c->shiftp,
-                            &c->xchild, &c->xmaps, &c->xphase, &c->adam_m, &c->adam_v, &c->gidx})
+                            &c->xchild, &c->xmaps, &c->xphase, &c->adam_m, &c->adam_v, &c->gidx, &c->lrpart})
         (void)hipFree(b->p);
     if (c->pinned.p) (void)hipHostFree(c->pinned.p);
     if (c->lane2) (void)hipStreamSynchronize(c->lane2);

@@ -101,14 +101,19 @@ __global__ __launch_bounds__(256) void k_zero_outside(float2* rho, const uint8_t
 
 // Per-individual step size of the support-projected refinement:
 // lr_p = lr_rel * sqrt(sum_{S} |rho|^2 / |S|) (0 for an empty support), summed
-// in double in a fixed order (one workgroup per individual).
-__global__ __launch_bounds__(256) void k_support_scale(const float2* rho, const uint8_t* supp, int np, float lr_rel,
-                                                       float* lrp) {
-    const int p = blockIdx.x;
-    const float2* r = rho + (size_t)p * np;
-    const uint8_t* s = supp + (size_t)p * np;
+// in double in a fixed order: SUPPORT_CHUNKS workgroups per individual each sum
+// a contiguous chunk (k_support_partials), then one thread per individual adds
+// the chunks' sums in order (k_support_lr).
+constexpr int SUPPORT_CHUNKS = 64;
+__global__ __launch_bounds__(256) void k_support_partials(const float2* __restrict__ rho,
+                                                          const uint8_t* __restrict__ supp, int np,
+                                                          double* __restrict__ part) {
+    const int p = blockIdx.x / SUPPORT_CHUNKS, j = blockIdx.x % SUPPORT_CHUNKS;
+    const int len = np / SUPPORT_CHUNKS;
+    const float2* r = rho + (size_t)p * np + (size_t)j * len;
+    const uint8_t* s = supp + (size_t)p * np + (size_t)j * len;
     double e = 0, cnt = 0;
-    for (int i = threadIdx.x; i < np; i += 256)
+    for (int i = threadIdx.x; i < len; i += 256)
         if (s[i]) {
             e += (double)r[i].x * r[i].x + (double)r[i].y * r[i].y;
             cnt += 1;
@@ -124,7 +129,21 @@ __global__ __launch_bounds__(256) void k_support_scale(const float2* rho, const 
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) lrp[p] = sc[0] > 0 ? (float)(lr_rel * sqrt(se[0] / sc[0])) : 0.f;
+    if (threadIdx.x == 0) {
+        part[2 * (size_t)blockIdx.x] = se[0];
+        part[2 * (size_t)blockIdx.x + 1] = sc[0];
+    }
+}
+__global__ __launch_bounds__(64) void k_support_lr(const double* __restrict__ part, int P, float lr_rel,
+                                                   float* __restrict__ lrp) {
+    const int p = blockIdx.x * 64 + threadIdx.x;
+    if (p >= P) return;
+    double e = 0, cnt = 0;
+    for (int j = 0; j < SUPPORT_CHUNKS; ++j) {
+        e += part[2 * ((size_t)p * SUPPORT_CHUNKS + j)];
+        cnt += part[2 * ((size_t)p * SUPPORT_CHUNKS + j) + 1];
+    }
+    lrp[p] = cnt > 0 ? (float)(lr_rel * sqrt(e / cnt)) : 0.f;
 }
 
 // Row pass of one refinement step, fused: the adjoint row transform of the

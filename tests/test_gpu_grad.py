@@ -137,6 +137,23 @@ def test_refinement_step_follows_the_gradient(hip, n, kind):
             assert size < 1e-3
 
 
+def test_refinement_gradient_pass_in_a_same_plan_variant_is_bit_identical(hip):
+    """The refinement's gradient pass runs in the handle's column variant when
+    that variant holds the default radix plan (at 512^2 the tuned variant 6:
+    8 columns in a skewed tile); its results equal the default geometry's bit
+    for bit."""
+    amp, rho, supp = case(512, 3, seed=27)
+    out = []
+    for cv in (0, 6):
+        be = hip.Backend(amp, capacity=3)
+        be.set_variants(cv, 0)
+        be.set_population(rho, supp)
+        be.refine_support(5, lr_rel=2e-2)
+        out.append(be.get_population(support=False))
+        be.close()
+    assert np.array_equal(out[0], out[1])
+
+
 @pytest.mark.parametrize("n", [128, 512])
 def test_support_refinement_matches_torch_adam(hip, n):
     """spr_refine_support (the GRAD operation) = torch.optim.Adam with one group
