@@ -165,7 +165,9 @@ struct AdamRowArgs {
 };
 
 // In the geometry of row variant RV; the densities are loaded with the field,
-// before the adjoint transform, so that their latency hides behind it.
+// before the adjoint transform, so that their latency hides behind it. MASKED:
+// the densities outside the support are 0 (the support refinement zeroes them
+// before its first step and never moves them), so only support pixels are read.
 template <int N, int RV, bool MASKED>
 __global__ __launch_bounds__((RowVariant<N, RV>::G::ROW_THREADS)) void k_row_adam(AdamRowArgs a) {
     using G = typename RowVariant<N, RV>::G;
@@ -188,10 +190,17 @@ __global__ __launch_bounds__((RowVariant<N, RV>::G::ROW_THREADS)) void k_row_ada
     const RowAccT<G::ROW_PAD_SHIFT> acc{lds + rl * G::RS};
     const float lr = MASKED ? a.lrp[field] : a.lr;
     float2 v[V], xs[V];
+    uint8_t sb[V];
 #pragma unroll
     for (int r = 0; r < V; ++r) {
         v[r] = work[off + r * T];
-        xs[r] = x[off + r * T];
+        if constexpr (MASKED) {
+            sb[r] = a.supp[blk + off + r * T];
+            xs[r] = sb[r] ? x[off + r * T] : make_float2(0.f, 0.f);
+        } else {
+            sb[r] = 1;
+            xs[r] = x[off + r * T];
+        }
     }
     __syncthreads();
     fft_line_p<PL, -1, WV>(v, acc, tw, t);
@@ -200,7 +209,7 @@ __global__ __launch_bounds__((RowVariant<N, RV>::G::ROW_THREADS)) void k_row_ada
         const int i = off + q * T;
         const float2 gi = make_float2(v[q].x * 2.f, v[q].y * 2.f);
         float2 xi = xs[q];
-        if (!MASKED || a.supp[blk + i]) {
+        if (!MASKED || sb[q]) {
             float2 mi = m[i], vi = v2[i];
             mi = make_float2(a.b1 * mi.x + (1.f - a.b1) * gi.x, a.b1 * mi.y + (1.f - a.b1) * gi.y);
             vi = make_float2(a.b2 * vi.x + (1.f - a.b2) * gi.x * gi.x, a.b2 * vi.y + (1.f - a.b2) * gi.y * gi.y);
