@@ -900,9 +900,8 @@ int queue_adam_steps(spr_ctx* c, int steps, float lr, const float* lrp, int kind
     const size_t n = (size_t)c->n * c->n * P;
     SPR_TRY2(ensure(c, c->adam_m, n * sizeof(float2)));
     SPR_TRY2(ensure(c, c->adam_v, n * sizeof(float2)));
-    // The moments start at 0: the first step writes them without reading them,
-    // so they need no clearing (and the masked steps never touch them outside
-    // the supports).
+    SPR_TRY2(fill_u32(c, (unsigned*)c->adam_m.p, 0u, 2 * n));
+    SPR_TRY2(fill_u32(c, (unsigned*)c->adam_v.p, 0u, 2 * n));
     FieldBatch fwd{c->work, nullptr, c->tw, 1.f, c->rho};
     SPR_TRY2(fft_rows(c, +1, fwd, P));
     const float b1 = 0.9f, b2 = 0.999f;
@@ -913,7 +912,7 @@ int queue_adam_steps(spr_ctx* c, int steps, float lr, const float* lrp, int kind
         p1 *= b1;
         p2 *= b2;
         AdamRowArgs ar{c->rho, (float2*)c->adam_m.p, (float2*)c->adam_v.p, c->work, c->supp, lrp, c->tw,
-                       lr, b1, b2, 1e-8f, 1.f - p1, 1.f - p2, s == 1 ? 1 : 0};
+                       lr, b1, b2, 1e-8f, 1.f - p1, 1.f - p2};
         SPR_TRY2(row_adam(c, ar, lrp != nullptr, P));
     }
     return SPR_OK;

@@ -162,7 +162,6 @@ struct AdamRowArgs {
     const float* lrp;
     const float2* tw;
     float lr, b1, b2, eps, bc1, bc2;
-    int first;   // the first step of a refinement: the moments start at 0 and are not read
 };
 
 // In the geometry of row variant RV; the densities are loaded with the field,
@@ -211,11 +210,7 @@ __global__ __launch_bounds__((RowVariant<N, RV>::G::ROW_THREADS)) void k_row_ada
         const float2 gi = make_float2(v[q].x * 2.f, v[q].y * 2.f);
         float2 xi = xs[q];
         if (!MASKED || sb[q]) {
-            float2 mi = make_float2(0.f, 0.f), vi = make_float2(0.f, 0.f);
-            if (!a.first) {
-                mi = m[i];
-                vi = v2[i];
-            }
+            float2 mi = m[i], vi = v2[i];
             mi = make_float2(a.b1 * mi.x + (1.f - a.b1) * gi.x, a.b1 * mi.y + (1.f - a.b1) * gi.y);
             vi = make_float2(a.b2 * vi.x + (1.f - a.b2) * gi.x * gi.x, a.b2 * vi.y + (1.f - a.b2) * gi.y * gi.y);
             xi.x -= lr * (mi.x / a.bc1) / (sqrtf(vi.x / a.bc2) + a.eps);

@@ -174,14 +174,14 @@ def test_support_refinement_matches_torch_adam(hip, n):
     assert not np.array_equal(got[0][supp[0] == 1], rho[0][supp[0] == 1])
 
 
-def test_refinement_starts_its_moments_from_zero_beyond_2g_elements(hip):
-    """The Adam moments of a refinement start from 0 at every call (its first
-    step writes them without reading them), also when the moment arrays hold
-    more than 2^31 32-bit words (1024^2 with 1100 fields: 2 N^2 P = 2.3e9).
-    Three individuals replicated on the device over 1100 slots are refined
-    twice from the same start (the first call leaves non-zero moments behind);
-    the second call's results equal those of a fresh three-field handle bit for
-    bit (the per-field arithmetic does not depend on the population size)."""
+def test_refinement_clears_its_moments_beyond_2g_elements(hip):
+    """The Adam moments of a refinement are cleared at every call, also when the
+    moment arrays hold more than 2^31 32-bit words (1024^2 with 1100 fields:
+    2 N^2 P = 2.3e9). Three individuals replicated on the device over 1100
+    slots are refined twice from the same start (the first call leaves
+    non-zero moments behind); the second call's results equal those of a
+    fresh three-field handle bit for bit (the per-field arithmetic does not
+    depend on the population size)."""
     n, P = 1024, 1100
     amp, rho, supp = case(n, 3, seed=41)
     print(f"refinement of {P} fields of {n}^2 (moments {2 * n * n * P} words)", flush=True)
