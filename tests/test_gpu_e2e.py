@@ -18,9 +18,14 @@ then shrink-wrap, alignment and averaging, Crossover, the sequence on parents
 and children, replacement of a parent by a better child) on the GPU against
 the FP32 oracle's restatement of the same loop (oracle/memetic.py), whose
 per-seed results are the committed fixtures tests/golden/memetic_*.json
-(written by tests/golden/make_memetic_fixtures.py). Same bars: medians of the
-final best error within 10 % and of the real-space error within 0.05,
-two-sample KS p > 0.01 for both.
+(written by tests/golden/make_memetic_fixtures.py): 128^2 polygon patterns with
+population 8 and the plain_small schedule, and 512^2 ellipsoid patterns (the
+plain drop-in configuration's patterns) with population 16 and the plain_512
+schedule (RAAR 400 + HIO 100 + ER 50). Bars: medians of the final best error
+within 10 %, two-sample KS p > 0.01 for the best and the real-space errors, the
+per-seed real-space differences with median within 0.02 and a Wilcoxon
+signed-rank p > 0.01, and the unpaired real-space medians within 0.05 where the
+median of the 32 draws is stable (see the test).
 """
 import os
 
@@ -96,11 +101,11 @@ def _fixture_cases():
     return mod
 
 
-@pytest.mark.parametrize("n", [128])
+@pytest.mark.parametrize("n", [128, 512])
 def test_memetic_generation_distribution_matches_oracle(n):
     import json
 
-    from scipy.stats import ks_2samp
+    from scipy.stats import ks_2samp, wilcoxon
 
     from s_spi_spring_amd import Pattern
     from s_spi_spring_amd.mpr import MPR
@@ -124,8 +129,25 @@ def test_memetic_generation_distribution_matches_oracle(n):
               f"{g_rs[-1]:.4f} oracle {c_rs[-1]:.4f}", flush=True)
     g_err, g_rs, c_err, c_rs = map(np.array, (g_err, g_rs, c_err, c_rs))
     p_err, p_rs = ks_2samp(g_err, c_err).pvalue, ks_2samp(g_rs, c_rs).pvalue
+    # Both runs of a seed start from the same pattern and seed, so the per-seed
+    # differences of the real-space error are the sharper measure: their median
+    # within 0.02 and a Wilcoxon signed-rank test not rejecting a zero shift.
+    d_rs = g_rs - c_rs
+    p_pair = wilcoxon(g_rs, c_rs).pvalue if np.any(d_rs != 0) else 1.0
+    # The unpaired medians of the real-space error are compared only where the
+    # median of 32 draws is stable: its bootstrap deviation over the oracle's
+    # sample at most 0.02 (128^2: the errors lie in 0.08 - 0.15 between the
+    # quartiles; 512^2: in 0.59 - 0.82, where the median of 32 moves by about
+    # 0.04 between samples).
+    rng = np.random.default_rng(0)
+    sd_med = float(np.std([np.median(rng.choice(c_rs, len(c_rs))) for _ in range(2000)]))
     print(f"n={n}: median best error gpu {np.median(g_err):.4f} oracle {np.median(c_err):.4f}; median real space "
-          f"gpu {np.median(g_rs):.4f} oracle {np.median(c_rs):.4f}; KS p {p_err:.3f} / {p_rs:.3f}")
+          f"gpu {np.median(g_rs):.4f} oracle {np.median(c_rs):.4f} (bootstrap deviation of the median "
+          f"{sd_med:.3f}); per-seed real-space differences: median {np.median(d_rs):+.4f}, Wilcoxon p "
+          f"{p_pair:.3f}; KS p {p_err:.3f} / {p_rs:.3f}")
     assert abs(np.median(g_err) - np.median(c_err)) <= 0.10 * np.median(c_err)
-    assert abs(np.median(g_rs) - np.median(c_rs)) <= 0.05
     assert p_err > 0.01 and p_rs > 0.01
+    assert abs(np.median(d_rs)) <= 0.02
+    assert p_pair > 0.01
+    if sd_med <= 0.02:
+        assert abs(np.median(g_rs) - np.median(c_rs)) <= 0.05
