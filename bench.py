@@ -230,6 +230,24 @@ def _ratio(a, b):
     return a / b if a and b else None
 
 
+def ttr_paired(plain, grad):
+    """Per-seed comparison of two legs run on the same seeds (the seeds both
+    reach the target): the median of the per-seed time ratios plain / grad, the
+    seeds where each leg is faster, and the two-sided Wilcoxon signed-rank test
+    of the per-seed time differences (p near 1: no systematic difference)."""
+    from scipy.stats import wilcoxon
+    tp = {r["seed"]: r["time_to_target"] for r in plain}
+    tg = {r["seed"]: r["time_to_target"] for r in grad}
+    both = sorted(s for s in tp if tp[s] is not None and tg.get(s) is not None)
+    a = np.array([tp[s] for s in both])
+    b = np.array([tg[s] for s in both])
+    d = a - b
+    p = float(wilcoxon(a, b).pvalue) if len(both) and np.any(d != 0) else None
+    return {"seeds_both_reached": len(both),
+            "median_ratio_plain_over_grad": float(np.median(a / b)) if len(both) else None,
+            "grad_faster": int(np.sum(d > 0)), "plain_faster": int(np.sum(d < 0)), "wilcoxon_p": p}
+
+
 def time_to_rf_fields(seeds, generations, target, log, group_seeds=8):
     """The time-to-R_F half of the headline metric for the default bench line:
     batched+gradient against batched plain over `seeds` seeds (the gradient
@@ -244,6 +262,7 @@ def time_to_rf_fields(seeds, generations, target, log, group_seeds=8):
                         "on the Poisson likelihood after each RAAR block",
             "batched_grad": sg, "batched_plain": sp, "plain_8_per_launch": dict(s8, seeds=group_seeds),
             "speedup_grad_vs_batched_plain": _ratio(sp["median_time_s"], sg["median_time_s"]),
+            "paired_grad_vs_batched_plain": ttr_paired(runs["plain"], runs["grad"]),
             "speedup_grad_vs_8_per_launch": _ratio(s8["median_time_s"], sg["median_time_s"]),
             "median_noise_floor": float(np.median([r["noise_floor"] for r in runs["plain"]])),
             "sw_threshold": workloads.SW_THRESHOLD_512}
@@ -272,7 +291,8 @@ def time_to_rf_study(seeds, generations, target, log, threshold=None, groups=())
            "generations_max": generations, "config": {"workload": "ellipsoids 512^2 pattern, 1e9 photons, "
                                                       "population 64, RAAR400+HIO100+ER50 per generation in "
                                                       "25-iteration blocks with shrink-wrap"},
-           "plain": sp, "grad": sg, "speedup_grad_vs_plain": speedup, **grouped,
+           "plain": sp, "grad": sg, "speedup_grad_vs_plain": speedup,
+           "paired_grad_vs_plain": ttr_paired(runs["plain"], runs["grad"]), **grouped,
            "sw_threshold": threshold if threshold is not None else workloads.SW_THRESHOLD_512,
            "median_noise_floor": float(np.median([r["noise_floor"] for r in runs["plain"]])),
            "per_seed": {f: [{k: r[k] for k in ("seed", "time_to_target", "generation", "fourier_error",
