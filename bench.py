@@ -79,9 +79,17 @@ def cpu_baseline(amp, rho, supp, alg, beta, iters, log):
     except OSError:
         pass
     log(f"cpu baseline: {P} individuals x {iters} {alg} iterations on {cores} threads ({cpu}): {dt:.2f} s")
+    # For information, the plain FP32 oracle and the FP64 oracle on small samples.
+    info = {}
+    for name, arr, k, its in (("plain_fp32_oracle", sample, P, 5), ("fp64_oracle", sample.astype(np.complex128), 16, 5)):
+        t0 = time.perf_counter()
+        orc.iterate(alg, arr[:k], supp[:k], amp.astype(arr.real.dtype), its, beta, 0.0, nthreads=cores)
+        t = time.perf_counter() - t0
+        info[name] = {"value": k * its / t, "sample": f"{k} individuals x {its} {alg} iterations"}
+        log(f"{name}: {k * its / t:.1f} individual-iterations/s")
     return {"value": P * iters / dt, "unit": "individual-iters/s", "cores": cores, "kind": "port",
             "sample": f"FP32 oracle (throughput port, AVX2), {P} individuals x {iters} {alg} iterations at "
-                      f"{amp.shape[0]}^2 ({cpu})"}
+                      f"{amp.shape[0]}^2 ({cpu})", **info}
 
 
 def copy_calibration(mib, log):
