@@ -243,14 +243,17 @@ def ttr_paired(plain, grad):
     reach the target): the median of the per-seed time ratios plain / grad, the
     seeds where each leg is faster, and the two-sided Wilcoxon signed-rank test
     of the per-seed time differences (p near 1: no systematic difference)."""
-    from scipy.stats import wilcoxon
+    try:
+        from scipy.stats import wilcoxon
+    except ImportError:   # the test statistic is then left out
+        wilcoxon = None
     tp = {r["seed"]: r["time_to_target"] for r in plain}
     tg = {r["seed"]: r["time_to_target"] for r in grad}
     both = sorted(s for s in tp if tp[s] is not None and tg.get(s) is not None)
     a = np.array([tp[s] for s in both])
     b = np.array([tg[s] for s in both])
     d = a - b
-    p = float(wilcoxon(a, b).pvalue) if len(both) and np.any(d != 0) else None
+    p = float(wilcoxon(a, b).pvalue) if wilcoxon is not None and len(both) and np.any(d != 0) else None
     return {"seeds_both_reached": len(both),
             "median_ratio_plain_over_grad": float(np.median(a / b)) if len(both) else None,
             "grad_faster": int(np.sum(d > 0)), "plain_faster": int(np.sum(d < 0)), "wilcoxon_p": p}
