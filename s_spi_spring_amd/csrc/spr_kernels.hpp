@@ -84,16 +84,13 @@ struct GeoT {
     // x RPW rows evenly over the LDS banks when the tile is transposed.
     static constexpr int RPW = (C >= 64) ? 1 : 64 / C;
     static constexpr int CST = ((N + N / 8) / (2 * RPW) + 1) * (2 * RPW) + RPW;
-    // Two column workgroups per CU when their LDS fits (160 KB): waves per SIMD to target.
+    // LDS of a column-pass workgroup: the tile (float2), the twiddle table unless it
+    // is read from global memory, and 512 bytes for the loss pass's reduction.
     static constexpr int COL_TILE = COL_WAVE ? C * CST : CSK ? (N + (N >> CSK)) * C : N * CS;   // float2 of the column tile
     static constexpr int COL_LDS = (COL_TILE + (TWG ? 0 : TwTableP<PL>::SIZE)) * 8 + 512;
 #ifndef SPR_COL_WG_PER_CU
 #define SPR_COL_WG_PER_CU 8
 #endif
-    // Row workgroups per CU allowed by LDS (4 waves each, one per SIMD): the
-    // register budget the row pass is compiled for.
-    static constexpr int ROW_LDS = (RW * RS + TwTableP<PL>::SIZE) * 8;
-    static constexpr int ROW_WAVES_PER_EU = (160 * 1024 / ROW_LDS) < 8 ? (160 * 1024 / ROW_LDS) : 8;
     // Workgroups per CU: as many as LDS and the 32 wave slots of a CU allow,
     // at most SPR_COL_WG_PER_CU; the register budget follows from it.
     static constexpr int COL_WG_LDS = (160 * 1024) / COL_LDS;
