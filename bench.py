@@ -445,7 +445,7 @@ def batch_bench(args, log):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--pop", type=int, default=64)
