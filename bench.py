@@ -41,14 +41,20 @@ COUNTER_CMD = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 90 rocprofv3 --
                "--no-cpu --prewarm 0 --ttr-seeds 0 --calibrate {mib}{extra}")
 
 
-def algorithmic_bytes(n, P, alg):
+def algorithmic_bytes(n, P, alg, recomputed_state=False):
     """HBM bytes per launch of the two iteration kernels (algorithmic, no re-reads):
     column pass 16 B/px (+ the amplitude, 4 B/px once per launch); row update pass
-    32.125 B/px for HIO/RAAR, 16.125 B/px for ER (the field read and written, the
-    state rho_n read and written for HIO/RAAR, the support as one bit per pixel)."""
+    for HIO/RAAR 32.125 B/px with the state stored (the field read and written,
+    rho_n read and written, the support as one bit per pixel) or 24.125 B/px with
+    it recomputed from its row transform (the projected field and the state's row
+    transform read, the new row transform written, the support bits); ER 16.125
+    B/px in both forms."""
     npx = n * n
     col = 16 * npx * P + 4 * npx
-    row = (32.125 if alg in ("HIO", "RAAR") else 16.125) * npx * P
+    if alg in ("HIO", "RAAR"):
+        row = (24.125 if recomputed_state else 32.125) * npx * P
+    else:
+        row = 16.125 * npx * P
     return col, row
 
 
@@ -417,10 +423,13 @@ def batch_bench(args, log):
     dt = time.perf_counter() - t0
     ind_iters = args.steps * args.batch_block * b.total
     value = ind_iters / dt
-    # algorithmic bytes of one iteration of the whole batch: 48.125 B/px per
-    # individual (16 column + 32.125 row) and each pattern's amplitude once per
-    # column pass
-    it_bytes = sum(48.125 * n * n * P + 4.0 * n * n for n in sizes)
+    # algorithmic bytes of one iteration of the whole batch: per individual 16
+    # B/px of column pass and 32.125 (state stored) or 24.125 (recomputed) of
+    # row pass, by the size class's row variant; each pattern's amplitude once
+    # per column pass
+    per_px = {n: 16 + (24.125 if hipcore.row_variant_recomputes_state(n, hipcore.default_variants(n)[1]) else 32.125)
+              for n in set(sizes)}
+    it_bytes = sum(per_px[n] * n * n * P + 4.0 * n * n for n in sizes)
     frac = it_bytes * (value / b.total) / 1e9 / HBM_PEAK_GBS
     log(f"{value:.0f} individual-iterations/s; whole iteration {it_bytes * value / b.total / 1e9:.0f} GB/s "
         f"algorithmic ({frac:.3f} of HBM peak)")
@@ -563,7 +572,8 @@ def main():
     value = ind_iters / dt
     chunk_bytes, chunk = be.chunk()
     fused = be.fused()[1]
-    col_b, row_b = algorithmic_bytes(n, min(P, chunk), args.alg)   # per launch (one chunk)
+    recomputed = hipcore.row_variant_recomputes_state(n, be.variants()[1])
+    col_b, row_b = algorithmic_bytes(n, min(P, chunk), args.alg, recomputed)   # per launch (one chunk)
     iter_bytes = (col_b + row_b) / min(P, chunk)
     iteration_frac = value * iter_bytes / 1e9 / HBM_PEAK_GBS
     if not fused:
@@ -627,6 +637,8 @@ def main():
                                f"blocks of {args.block} iterations, population resident in HBM",
                    "n": n, "population": P, "alg": args.alg, "iterations_per_step": args.block,
                    "kernel_variants": dict(zip(("column", "row"), cvars)),
+                   "state": "recomputed" if recomputed else "stored",
+                   "algorithmic_bytes_per_px": round(iter_bytes / (n * n), 3),
                    "chunk_individuals": min(P, chunk), "on_chip_blocks": fused},
         "roofline": roofline,
         "cpu_baseline": cpu,

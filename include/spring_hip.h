@@ -305,9 +305,16 @@ int spr_stream(spr_ctx* ctx, void** stream);
  * write the same layout. Width, threads and prefetch run the same operations
  * (up to the compiler's contraction of multiply-adds); another radix plan
  * changes the rounding of the transforms. Every variant meets the same
- * accuracy bar against the oracle.
+ * accuracy bar against the oracle. Some row variants recompute the HIO/RAAR
+ * state rho_n in each iteration from its row transform, which the iteration
+ * keeps anyway, instead of reading and writing it (8 fewer bytes per pixel
+ * and iteration for one more row transform); the column pass then writes its
+ * projection to a second field buffer. The recomputed state agrees with the
+ * stored one to float rounding.
  *
  * spr_variants: number of column and row variants of size n.
+ * spr_row_variant_info: whether row variant `variant` of size n recomputes
+ *   the state (1) or stores it (0).
  * spr_autotune: times `reps` iterations (a column pass, then a RAAR row
  *   pass) with every variant on the handle's resident population (densities
  *   and supports are left unchanged): each column variant with the handle's
@@ -322,6 +329,7 @@ int spr_stream(spr_ctx* ctx, void** stream);
  *   run on the target GPU, spr_tuned.hpp).
  * spr_batch_autotune: spr_autotune on every size class of a batch. */
 int spr_variants(int n, int* col_count, int* row_count);
+int spr_row_variant_info(int n, int variant, int* recomputes_state);
 
 /* Chunked iteration blocks. No counterpart in the reference. An ER/HIO/RAAR
  * block runs over the population in chunks whose densities, fields and

@@ -219,11 +219,12 @@ template <int N> int launch_row(spr_ctx* c, int mode, const RowArgs& a, int cnt,
 }
 
 // Column passes: the projection pass in every variant, the plain transforms
-// (2-D FT / IFT, off the iteration path) in variant 0.
-template <int N, int CV> int launch_col_v(spr_ctx* c, int mode, float2* f, int cnt, hipStream_t s, int first) {
+// (2-D FT / IFT, off the iteration path) in variant 0; in place, or into `out`.
+template <int N, int CV>
+int launch_col_v(spr_ctx* c, int mode, float2* f, int cnt, hipStream_t s, int first, float2* out = nullptr) {
     using G = typename ColVariant<N, CV>::G;
     if (G::TW_GLOBAL && !c->twtab[CV]) return fail(c, SPR_ERR_STATE, "twiddle table missing");
-    const ColArgs a{f, c->amp, c->tw, c->amp_t, c->amp_of ? c->amp_of + first : nullptr, c->twtab[CV]};
+    const ColArgs a{f, c->amp, c->tw, c->amp_t, c->amp_of ? c->amp_of + first : nullptr, c->twtab[CV], out ? out : f};
     const dim3 grid((unsigned)((size_t)cnt * (N / G::COL_STRIP))), block(G::COL_THREADS);
     switch (mode) {
         case COL_PROJ: hipLaunchKernelGGL((k_col<N, COL_PROJ, CV>), grid, block, 0, s, a); break;
@@ -257,30 +258,30 @@ template <int N, int CV = 0> int build_twtabs(spr_ctx* c) {
     return SPR_OK;
 }
 
-template <int N> int launch_col(spr_ctx* c, int mode, float2* f, int cnt, hipStream_t s, int first) {
-    if (mode != COL_PROJ) return launch_col_v<N, 0>(c, mode, f, cnt, s, first);
+template <int N> int launch_col(spr_ctx* c, int mode, float2* f, int cnt, hipStream_t s, int first, float2* out) {
+    if (mode != COL_PROJ) return launch_col_v<N, 0>(c, mode, f, cnt, s, first, out);
     switch (c->colvar) {
-        case 0: return launch_col_v<N, 0>(c, mode, f, cnt, s, first);
+        case 0: return launch_col_v<N, 0>(c, mode, f, cnt, s, first, out);
         case 1:
-            if constexpr (ColVariant<N, 1>::valid) return launch_col_v<N, 1>(c, mode, f, cnt, s, first);
+            if constexpr (ColVariant<N, 1>::valid) return launch_col_v<N, 1>(c, mode, f, cnt, s, first, out);
             break;
         case 2:
-            if constexpr (ColVariant<N, 2>::valid) return launch_col_v<N, 2>(c, mode, f, cnt, s, first);
+            if constexpr (ColVariant<N, 2>::valid) return launch_col_v<N, 2>(c, mode, f, cnt, s, first, out);
             break;
         case 3:
-            if constexpr (ColVariant<N, 3>::valid) return launch_col_v<N, 3>(c, mode, f, cnt, s, first);
+            if constexpr (ColVariant<N, 3>::valid) return launch_col_v<N, 3>(c, mode, f, cnt, s, first, out);
             break;
         case 4:
-            if constexpr (ColVariant<N, 4>::valid) return launch_col_v<N, 4>(c, mode, f, cnt, s, first);
+            if constexpr (ColVariant<N, 4>::valid) return launch_col_v<N, 4>(c, mode, f, cnt, s, first, out);
             break;
         case 5:
-            if constexpr (ColVariant<N, 5>::valid) return launch_col_v<N, 5>(c, mode, f, cnt, s, first);
+            if constexpr (ColVariant<N, 5>::valid) return launch_col_v<N, 5>(c, mode, f, cnt, s, first, out);
             break;
         case 6:
-            if constexpr (ColVariant<N, 6>::valid) return launch_col_v<N, 6>(c, mode, f, cnt, s, first);
+            if constexpr (ColVariant<N, 6>::valid) return launch_col_v<N, 6>(c, mode, f, cnt, s, first, out);
             break;
         case 7:
-            if constexpr (ColVariant<N, 7>::valid) return launch_col_v<N, 7>(c, mode, f, cnt, s, first);
+            if constexpr (ColVariant<N, 7>::valid) return launch_col_v<N, 7>(c, mode, f, cnt, s, first, out);
             break;
     }
     return fail(c, SPR_ERR_ARG, "bad column variant");
@@ -305,9 +306,10 @@ template <int N> int launch_col(spr_ctx* c, int mode, float2* f, int cnt, hipStr
 int row_n(spr_ctx* c, int mode, const RowArgs& a, int cnt, hipStream_t s) {
     SPR_DISPATCH_N(c, launch_row, c, mode, a, cnt, s);
 }
-// Column pass over fields f[0, cnt), the individuals [first, first + cnt).
-int col_n(spr_ctx* c, int mode, float2* f, int cnt, hipStream_t s, int first = 0) {
-    SPR_DISPATCH_N(c, launch_col, c, mode, f, cnt, s, first);
+// Column pass over fields f[0, cnt), the individuals [first, first + cnt),
+// in place or into out[0, cnt).
+int col_n(spr_ctx* c, int mode, float2* f, int cnt, hipStream_t s, int first = 0, float2* out = nullptr) {
+    SPR_DISPATCH_N(c, launch_col, c, mode, f, cnt, s, first, out);
 }
 
 // Timed passes (spr_timing) over `cnt` individuals from the argument pointers
@@ -320,11 +322,11 @@ int row(spr_ctx* c, int mode, const RowArgs& a, int cnt = -1) {
     if (rc != SPR_OK) return rc;
     return timing_end(c);
 }
-int col(spr_ctx* c, int mode, float2* f, int cnt = -1, int first = 0) {
+int col(spr_ctx* c, int mode, float2* f, int cnt = -1, int first = 0, float2* out = nullptr) {
     const int kind = (mode == COL_PROJ) ? SPR_KIND_COL_PROJ : SPR_KIND_COL_OTHER;
     int rc = timing_begin(c, kind);
     if (rc != SPR_OK) return rc;
-    rc = col_n(c, mode, f, cnt < 0 ? c->P : cnt, c->stream, first);
+    rc = col_n(c, mode, f, cnt < 0 ? c->P : cnt, c->stream, first, out);
     if (rc != SPR_OK) return rc;
     return timing_end(c);
 }
@@ -339,7 +341,26 @@ RowArgs row_args(spr_ctx* c) {
     a.beta = 0.f;
     a.inv_nn = 1.0f / (float)((double)c->n * c->n);
     a.pc = cut_er_hio(0.f);
+    a.proj = nullptr;
+    a.inv_n = 1.0f / (float)c->n;
     return a;
+}
+
+// Whether row variant rv of size n recomputes rho_n from its row transform
+// (RowVariant::G::ROW_RECOMPUTE); false for variants that do not exist.
+template <int N, int RV = 0> bool row_recomputes_n(int rv) {
+    if constexpr (RV < MAX_VARIANTS) {
+        if constexpr (RowVariant<N, RV>::valid)
+            if (rv == RV) return RowVariant<N, RV>::G::ROW_RECOMPUTE;
+        return row_recomputes_n<N, RV + 1>(rv);
+    }
+    return false;
+}
+bool row_recomputes(int n, int rv) {
+#define SPR_RS_CASE(N_) if (n == N_) return row_recomputes_n<N_>(rv);
+    SPR_SIZES(SPR_RS_CASE)
+#undef SPR_RS_CASE
+    return false;
 }
 
 #define SPR_TRY(expr)                 \
@@ -596,14 +617,19 @@ int pack_support(spr_ctx* c, const uint8_t* supp, uint32_t* bits, int cnt, hipSt
 // pass, so that the column pass of each half runs beside the row pass of the
 // other. The halves share no memory (the twiddles and the amplitude are read
 // only); the context's stream waits for lane 2 at the end.
-int queue_iterate_lanes(spr_ctx* c, int mode, int niter, const RowArgs& a) {
+int queue_iterate_lanes(spr_ctx* c, int mode, int niter, const RowArgs& a0) {
     const int h = (c->P + 1) / 2, h2 = c->P - h;
     const size_t off = (size_t)h * c->n * c->n;
+    RowArgs a = a0;
+    if (row_recomputes(c->n, c->rowvar)) a.proj = c->work2;   // the column passes project into work2
     RowArgs b = a;
     b.rho += off;
     b.work += off;
     b.supp += off;
     b.sbits += off / 32;
+    if (b.proj) b.proj += off;
+    float2* pa = const_cast<float2*>(a.proj);
+    float2* pb = const_cast<float2*>(b.proj);
     hipStream_t s1 = c->stream, s2 = c->lane2;
     SPR_TRY(pack_support(c, c->supp, c->sbits, c->P, s1));
     SPR_TRY(row_n(c, ROW_FWD, a, h, s1));
@@ -611,13 +637,13 @@ int queue_iterate_lanes(spr_ctx* c, int mode, int niter, const RowArgs& a) {
     SPR_HIP(c, hipStreamWaitEvent(s2, c->ev_fork, 0));
     SPR_TRY(row_n(c, ROW_FWD, b, h2, s2));
     for (int i = 0; i < niter; ++i) {
-        SPR_TRY(col_n(c, COL_PROJ, a.work, h, s1));
-        SPR_TRY(col_n(c, COL_PROJ, b.work, h2, s2, h));
+        SPR_TRY(col_n(c, COL_PROJ, a.work, h, s1, 0, pa));
+        SPR_TRY(col_n(c, COL_PROJ, b.work, h2, s2, h, pb));
         SPR_TRY(row_n(c, mode, a, h, s1));
         SPR_TRY(row_n(c, mode, b, h2, s2));
     }
-    SPR_TRY(col_n(c, COL_PROJ, a.work, h, s1));
-    SPR_TRY(col_n(c, COL_PROJ, b.work, h2, s2, h));
+    SPR_TRY(col_n(c, COL_PROJ, a.work, h, s1, 0, pa));
+    SPR_TRY(col_n(c, COL_PROJ, b.work, h2, s2, h, pb));
     SPR_TRY(row_n(c, ROW_FINAL, a, h, s1));
     SPR_TRY(row_n(c, ROW_FINAL, b, h2, s2));
     SPR_HIP(c, hipEventRecord(c->ev_join, s2));
@@ -626,8 +652,10 @@ int queue_iterate_lanes(spr_ctx* c, int mode, int niter, const RowArgs& a) {
 }
 
 // Individuals per chunk of an iteration block: the population split into the
-// fewest equal chunks whose densities, fields and supports (17 bytes per pixel)
-// fit in c->chunk_bytes (at least one individual per chunk).
+// fewest equal chunks whose densities, fields and supports (17 bytes per pixel;
+// with the recomputed state the iterations touch the two field buffers and the
+// support bits instead, about as much) fit in c->chunk_bytes (at least one
+// individual per chunk).
 int chunk_individuals(const spr_ctx* c) {
     if (c->chunk_bytes == 0 || c->P <= 1) return std::max(c->P, 1);
     const size_t per = (size_t)17 * c->n * c->n;
@@ -641,10 +669,13 @@ int chunk_individuals(const spr_ctx* c) {
 // run chunk after chunk. Individuals are independent, so the chunks compute
 // exactly what one pass over all of them would; a chunk's working set stays in
 // the GPU's last-level cache (256 MB Infinity Cache on MI355X) across its
-// iterations instead of streaming the whole population from HBM per pass.
+// iterations instead of streaming the whole population from HBM per pass. With
+// a row variant that recomputes rho_n the column passes project into work2
+// (proj) and rho is read by the first row pass and written by the last only.
 int queue_iterate_block(spr_ctx* c, int mode, int niter, const RowArgs& a0) {
     const int chunk = chunk_individuals(c);
     const size_t np = (size_t)c->n * c->n;
+    const bool rs = row_recomputes(c->n, c->rowvar);
     SPR_TRY(pack_support(c, a0.supp, const_cast<uint32_t*>(a0.sbits), c->P, c->stream));
     for (int first = 0; first < c->P; first += chunk) {
         const int cnt = std::min(chunk, c->P - first);
@@ -653,12 +684,14 @@ int queue_iterate_block(spr_ctx* c, int mode, int niter, const RowArgs& a0) {
         a.work += first * np;
         a.supp += first * np;
         a.sbits += first * np / 32;
+        float2* pj = rs ? c->work2 + first * np : nullptr;
+        a.proj = pj;
         SPR_TRY(row(c, ROW_FWD, a, cnt));
         for (int i = 0; i < niter; ++i) {
-            SPR_TRY(col(c, COL_PROJ, a.work, cnt, first));
+            SPR_TRY(col(c, COL_PROJ, a.work, cnt, first, pj));
             SPR_TRY(row(c, mode, a, cnt));
         }
-        SPR_TRY(col(c, COL_PROJ, a.work, cnt, first));
+        SPR_TRY(col(c, COL_PROJ, a.work, cnt, first, pj));
         SPR_TRY(row(c, ROW_FINAL, a, cnt));
     }
     return SPR_OK;
@@ -1039,9 +1072,24 @@ VariantPair default_variants(int n) {
 // handle's current variants. Timing whole iterations, not one pass repeated,
 // keeps the caches as the iteration leaves them: a pass that reads a table from
 // global memory finds it evicted by the other pass, as it does in a block.
-int time_iteration(spr_ctx* c, const RowArgs& ra, int reps, hipEvent_t e0, hipEvent_t e1, float* ms) {
+// The densities are not changed: a row variant that stores rho_n updates a
+// fresh copy of them in work2; one that recomputes it leaves them alone, and
+// its column passes project into work2.
+int time_iteration(spr_ctx* c, const RowArgs& ra0, int reps, hipEvent_t e0, hipEvent_t e1, float* ms) {
+    RowArgs ra = ra0;
+    float2* out = nullptr;
+    if (row_recomputes(c->n, c->rowvar)) {
+        ra.proj = c->work2;
+        out = c->work2;
+    } else {
+        const int np = c->n * c->n;
+        hipLaunchKernelGGL(k_copy_fields, dim3((np + 255) / 256, c->P), dim3(256), 0, c->stream, (const float2*)c->rho,
+                           (const int*)nullptr, c->work2, (const int*)nullptr, np);
+        SPR_HIP(c, hipGetLastError());
+        ra.rho = c->work2;
+    }
     auto once = [&]() {
-        SPR_TRY(col_n(c, COL_PROJ, c->work, c->P, c->stream));
+        SPR_TRY(col_n(c, COL_PROJ, c->work, c->P, c->stream, 0, out));
         return row_n(c, ROW_RAAR, ra, c->P, c->stream);
     };
     SPR_TRY(once());   // warm-up (code load, caches)
@@ -1058,21 +1106,16 @@ int time_iteration(spr_ctx* c, const RowArgs& ra, int reps, hipEvent_t e0, hipEv
 // Times every column and row variant of the handle's size on its resident
 // population and keeps the fastest: first the column variants, each in whole
 // iterations with the handle's row variant, then the row variants with the
-// chosen column variant. The densities and supports are not changed: the row
-// passes update a copy of the densities in work2 and the column passes
-// transform `work`, which every operation rewrites first.
+// chosen column variant. The densities and supports are not changed (see
+// time_iteration); the passes transform `work` and `work2`, which every
+// operation rewrites first.
 int autotune(spr_ctx* c, int reps, float* col_ms, float* row_ms) {
     if (c->P <= 0) return fail(c, SPR_ERR_ARG, "autotune needs resident individuals");
     if (reps < 1) return fail(c, SPR_ERR_ARG, "reps must be >= 1");
     int nc = 1, nr = 1;
     SPR_TRY(variant_counts(c->n, &nc, &nr));
-    const int np = c->n * c->n;
     SPR_HIP(c, hipSetDevice(c->device));
-    hipLaunchKernelGGL(k_copy_fields, dim3((np + 255) / 256, c->P), dim3(256), 0, c->stream, (const float2*)c->rho,
-                       (const int*)nullptr, c->work2, (const int*)nullptr, np);
-    SPR_HIP(c, hipGetLastError());
     RowArgs ra = row_args(c);
-    ra.rho = c->work2;
     ra.beta = 0.87f;
     ra.pc = cut_raar(0.f);
     const VariantPair keep{c->colvar, c->rowvar};
@@ -2164,6 +2207,14 @@ int spr_variants(int n, int* col_count, int* row_count) {
     if (variant_counts(n, &nc, &nr) != SPR_OK) return fail(nullptr, SPR_ERR_ARG, "unsupported grid size");
     if (col_count) *col_count = nc;
     if (row_count) *row_count = nr;
+    return SPR_OK;
+}
+
+int spr_row_variant_info(int n, int variant, int* recomputes_state) {
+    int nc = 0, nr = 0;
+    if (variant_counts(n, &nc, &nr) != SPR_OK) return fail(nullptr, SPR_ERR_ARG, "unsupported grid size");
+    if (variant < 0 || variant >= nr) return fail(nullptr, SPR_ERR_ARG, "variant out of range");
+    if (recomputes_state) *recomputes_state = row_recomputes(n, variant) ? 1 : 0;
     return SPR_OK;
 }
 

@@ -11,13 +11,21 @@
 //   tw   [N]       float2   exp(-2*pi*i*m/N)
 //
 // One iteration = two launches (the minimal number of HBM round trips for a field
-// that does not fit in one CU's LDS):
-//   column pass  work <- colIFFT( P_M( colFFT(work) ) )                     16 B/px
-//   row pass     rho' = rowIFFT(work)/N^2 ; rho <- update(rho', rho, S) ;
-//                work <- rowFFT(rho)                                  32.125 B/px
-// so HIO/RAAR move 48.125 B/px per individual-iteration; ER, which needs no
-// rho_n, moves 32.125 B/px and writes rho only at the end. (The support packing
-// reads 1 and writes 0.125 B/px once per block.)
+// that does not fit in one CU's LDS), in one of two forms chosen by the row-pass
+// variant:
+//   state stored (rho_n read and written every iteration)
+//     column pass  work <- colIFFT( P_M( colFFT(work) ) )                   16 B/px
+//     row pass     rho' = rowIFFT(work)/N^2 ; rho <- update(rho', rho, S) ;
+//                  work <- rowFFT(rho)                                32.125 B/px
+//   state recomputed (work holds rowFFT(rho_n); rho is read and written only at
+//   the ends of a block)
+//     column pass  proj <- colIFFT( P_M( colFFT(work) ) )                   16 B/px
+//     row pass     rho' = rowIFFT(proj)/N^2 ; rho_n = rowIFFT(work)/N ;
+//                  work <- rowFFT(update(rho', rho_n, S))             24.125 B/px
+// so HIO/RAAR move 48.125 or 40.125 B/px per individual-iteration (the second
+// form spends one more row transform to save the state's round trip); ER, which
+// needs no rho_n, moves 32.125 B/px in both. proj is the work2 buffer. (The
+// support packing reads 1 and writes 0.125 B/px once per block.)
 #pragma once
 
 #include <cstdint>
@@ -45,8 +53,10 @@ constexpr int pow2_floor(int x) { return x >= 2 ? 2 * pow2_floor(x / 2) : 1; }
 // 128-byte row segments), transformed one group after another through the same
 // C-column LDS tile; WPE > 0 sets the column pass's waves per SIMD (its
 // register budget) instead of deriving it from the workgroups a CU holds.
+// RST = 1 makes the row update passes recompute rho_n from its row transform
+// instead of reading and writing it (the second iteration form above).
 template <int N, class PL_ = RadixPlan<N>, int CCOLS = 0, int RTHREADS = 256, int RPF = -1, int PAD = 1, int TWG = 0,
-          int RSH = 3, int CSK = 0, int CGR = 1, int WPE = 0>
+          int RSH = 3, int CSK = 0, int CGR = 1, int WPE = 0, int RST = 0>
 struct GeoT {
     using PL = PL_;
     static constexpr bool TW_GLOBAL = TWG != 0;
@@ -55,6 +65,7 @@ struct GeoT {
     static constexpr int T = N / V;                       // threads per line
     static constexpr int ROW_THREADS = RTHREADS;
     static constexpr int ROW_PREFETCH = RPF;
+    static constexpr bool ROW_RECOMPUTE = RST != 0;      // rho_n recomputed from work
     static constexpr int RW = ROW_THREADS / T;            // rows per row-pass workgroup
     static constexpr int RS = N + (N >> RSH);             // padded LDS row length (float2)
 #ifndef SPR_COL_COLUMNS
@@ -179,6 +190,19 @@ SPR_ROW_VARIANT(512, 2, RadixPlan<512>, 0, 256, -1, 1, 0, 4)
 SPR_ROW_VARIANT(768, 3, RadixPlan<768>, 0, 256, -1, 1, 0, 5)
 SPR_ROW_VARIANT(1024, 5, RadixPlan<1024>, 0, 256, -1, 1, 0, 4)
 SPR_ROW_VARIANT(2048, 3, RadixPlan<2048>, 0, 256, -1, 1, 0, 4)
+// (plan, -, threads, prefetch, -, -, padding period, -, -, -, recomputed state):
+// the row update passes recompute rho_n from its row transform instead of
+// reading and writing it (one more row transform for 8 fewer bytes per pixel);
+// the column pass then writes its projection to a second field buffer
+SPR_ROW_VARIANT(256, 3, RadixPlan<256>, 0, 256, -1, 1, 0, 4, 0, 1, 0, 1)
+SPR_ROW_VARIANT(384, 3, RadixPlan<384>, 0, 256, -1, 1, 0, 5, 0, 1, 0, 1)
+SPR_ROW_VARIANT(512, 3, RadixPlan<512>, 0, 256, -1, 1, 0, 3, 0, 1, 0, 1)
+SPR_ROW_VARIANT(512, 4, RadixPlan<512>, 0, 256, -1, 1, 0, 4, 0, 1, 0, 1)
+SPR_ROW_VARIANT(768, 4, RadixPlan<768>, 0, 256, -1, 1, 0, 5, 0, 1, 0, 1)
+SPR_ROW_VARIANT(1024, 6, RadixPlan<1024>, 0, 256, -1, 1, 0, 4, 0, 1, 0, 1)
+SPR_ROW_VARIANT(1024, 7, Plan<1024, 16, 16, 8, 8>, 0, 256, -1, 1, 0, 3, 0, 1, 0, 1)
+SPR_ROW_VARIANT(2048, 4, RadixPlan<2048>, 0, 256, -1, 1, 0, 4, 0, 1, 0, 1)
+SPR_ROW_VARIANT(2048, 5, Plan<2048, 16, 16, 8, 16>, 0, 256, -1, 1, 0, 4, 0, 1, 0, 1)
 #undef SPR_COL_VARIANT
 #undef SPR_ROW_VARIANT
 
@@ -365,6 +389,8 @@ struct RowArgs {
     float beta;
     float inv_nn;  // 1/N^2 (exact for power-of-two N; within 1 ulp of dividing by N^2 otherwise)
     PhaseCut pc;
+    const float2* proj;  // column-projected fields, read by variants that recompute rho_n
+    float inv_n;         // 1/N
 };
 
 // Support bit of pixel i of a field: bit i&31 of word i>>5 (N*N and the
@@ -405,6 +431,8 @@ __global__ __launch_bounds__((RowVariant<N, RV>::G::ROW_THREADS)) void k_row(Row
     constexpr int V = G::V, T = G::T;
     constexpr int PREFETCH = G::ROW_PREFETCH >= 0 ? G::ROW_PREFETCH : SPR_ROW_PREFETCH;
     constexpr bool WV = T <= 64;   // every row inside one wavefront: no workgroup barriers
+    constexpr bool RS = G::ROW_RECOMPUTE;
+    static_assert(!RS || PREFETCH >= 1, "the recomputed state is transformed from its prefetched row");
     __shared__ float2 tw[TwTableP<PL>::SIZE];
     __shared__ float2 lds[G::RW * G::RS];
     const int tid = threadIdx.x;
@@ -429,7 +457,7 @@ __global__ __launch_bounds__((RowVariant<N, RV>::G::ROW_THREADS)) void k_row(Row
 #pragma unroll
         for (int q = 0; q < V; ++q) dst[off + q * T] = v[q];
     } else if constexpr (MODE == ROW_FINAL || MODE == ROW_INV_INPLACE) {
-        const float2* src = (MODE == ROW_FINAL) ? work : rho;
+        const float2* src = (MODE == ROW_FINAL) ? (RS ? a.proj + blk : work) : rho;
 #pragma unroll
         for (int r = 0; r < V; ++r) v[r] = src[off + r * T];
         fft_line_p<PL, -1, WV>(v, acc, tw, t);
@@ -437,14 +465,22 @@ __global__ __launch_bounds__((RowVariant<N, RV>::G::ROW_THREADS)) void k_row(Row
         for (int q = 0; q < V; ++q) rho[off + q * T] = make_float2(v[q].x * a.inv_nn, v[q].y * a.inv_nn);
     } else {
         // The state and the support may be loaded with the field, before the
-        // transform, so that their latency hides behind it.
+        // transform, so that their latency hides behind it. With RS the field
+        // comes from proj and the state is recomputed from its row transform
+        // (work): rho_n = rowIFFT(rowFFT(rho_n)) / N, within float rounding.
         float2 bn[V];
         uint8_t sn[V];
+        const float2* __restrict__ fin = RS ? a.proj + blk : work;
 #pragma unroll
         for (int r = 0; r < V; ++r) {
-            v[r] = ld_stream(work + off + r * T);
+            v[r] = ld_stream(fin + off + r * T);
             if constexpr (PREFETCH >= 2) sn[r] = support_bit(sbits, off + r * T);
-            if constexpr (MODE != ROW_ER && PREFETCH >= 1) bn[r] = ld_stream(rho + off + r * T);
+            if constexpr (MODE != ROW_ER && PREFETCH >= 1) bn[r] = ld_stream((RS ? work : rho) + off + r * T);
+        }
+        if constexpr (RS && MODE != ROW_ER) {
+            fft_line_p<PL, -1, WV>(bn, acc, tw, t);
+#pragma unroll
+            for (int r = 0; r < V; ++r) bn[r] = make_float2(bn[r].x * a.inv_n, bn[r].y * a.inv_n);
         }
         fft_line_p<PL, -1, WV>(v, acc, tw, t);
 #pragma unroll
@@ -470,7 +506,7 @@ __global__ __launch_bounds__((RowVariant<N, RV>::G::ROW_THREADS)) void k_row(Row
                     o = make_float2(a.beta * b.x + c * d.x, a.beta * b.y + c * d.y);
                 }
             }
-            if constexpr (MODE != ROW_ER) st_stream(rho + i, o);
+            if constexpr (MODE != ROW_ER && !RS) st_stream(rho + i, o);
             v[q] = o;
         }
         fft_line_p<PL, +1, WV>(v, acc, tw, t);
@@ -495,6 +531,7 @@ struct ColArgs {
     const float* amp_t; // the amplitudes transposed, amp_t[x*N + y] = amp[y*N + x]
     const int* amp_of;  // field -> pattern (NULL: every field belongs to pattern 0)
     const float2* twtab;  // the plan's twiddle table in global memory (variants with TW_GLOBAL)
+    float2* out;          // output fields (= f: in place)
 };
 
 // Offset of the amplitude of field p (block-uniform: one scalar load).
@@ -548,6 +585,7 @@ __attribute__((amdgpu_waves_per_eu((ColVariant<N, CV>::G::COL_WAVES_PER_EU)))) v
     const int x0 = (strip % TILES) * G::COL_STRIP;
     // Block-uniform bases with 32-bit per-thread offsets (SGPR base addressing).
     float2* __restrict__ f = a.f + (size_t)p * N * N + x0;
+    float2* __restrict__ fo = a.out + (size_t)p * N * N + x0;
     const int off = t * N + c;
     // Group g of the strip is columns [g*C, (g+1)*C). The loads are issued
     // group after group, so that the wait for group 0's values leaves the later
@@ -612,13 +650,13 @@ __attribute__((amdgpu_waves_per_eu((ColVariant<N, CV>::G::COL_WAVES_PER_EU)))) v
             if constexpr (MODE == COL_PROJ || MODE == COL_INV) fft_line_p<PL, -1>(vg[g], acc, tw, t);
             if constexpr (CG > 1) {
 #pragma unroll
-                for (int q = 0; q < V; ++q) st_stream(f + off + g * C + q * T * N, vg[g][q]);
+                for (int q = 0; q < V; ++q) st_stream(fo + off + g * C + q * T * N, vg[g][q]);
             }
         }
         if constexpr (CG > 1) return;
     }
 #pragma unroll
-    for (int q = 0; q < V; ++q) st_stream(f + off + q * T * N, v[q]);
+    for (int q = 0; q < V; ++q) st_stream(fo + off + q * T * N, v[q]);
 }
 
 }  // namespace spr

@@ -9,11 +9,11 @@ struct TunedVariants {
     int n, col, row;
 };
 constexpr TunedVariants kTuned[] = {
-    {256, 1, 2},
+    {256, 1, 3},
     {384, 1, 2},
-    {512, 6, 0},
-    {768, 6, 3},
-    {1024, 2, 1},
+    {512, 6, 3},
+    {768, 2, 4},
+    {1024, 4, 1},
     {2048, 2, 0},
 };
 

@@ -79,6 +79,7 @@ SIGNATURES = {
     "spr_set_chunk": (_I, [_VP, ctypes.c_longlong]),
     "spr_get_chunk": (_I, [_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I)]),
     "spr_autotune": (_I, [_VP, _I, _VP, _VP]),
+    "spr_row_variant_info": (_I, [_I, _I, ctypes.POINTER(_I)]),
     "spr_set_variants": (_I, [_VP, _I, _I]),
     "spr_get_variants": (_I, [_VP, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "spr_default_variants": (_I, [_I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
@@ -171,6 +172,16 @@ def variant_counts(n):
     if lib.spr_variants(int(n), ctypes.byref(c), ctypes.byref(r)) != 0:
         raise SpringHipError(lib.spr_last_error(None).decode())
     return c.value, r.value
+
+
+def row_variant_recomputes_state(n, variant):
+    """Whether row variant `variant` of size n recomputes rho_n from its row
+    transform instead of storing it (spr_row_variant_info)."""
+    lib = load()
+    r = ctypes.c_int()
+    if lib.spr_row_variant_info(int(n), int(variant), ctypes.byref(r)) != 0:
+        raise SpringHipError(lib.spr_last_error(None).decode())
+    return bool(r.value)
 
 
 def default_variants(n):

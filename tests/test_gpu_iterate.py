@@ -79,14 +79,17 @@ def test_twenty_iterations(hip, orc, alg, beta):
         assert e < 1e-4
 
 
-def test_two_lane_iteration_is_bit_identical(hip, monkeypatch):
+@pytest.mark.parametrize("row_variant", [2, 3])
+def test_two_lane_iteration_is_bit_identical(hip, monkeypatch, row_variant):
     """The pipelined two-stream form (SPR_ITERATE_LANES=2, read at context
-    creation) runs the same per-individual arithmetic as the one-stream form."""
+    creation) runs the same per-individual arithmetic as the one-stream form,
+    with the state stored (row variant 2 at 256^2) or recomputed (3)."""
     amp, rho, supp = case(256, 5, seed=11)
     out = []
     for lanes in ("1", "2"):
         monkeypatch.setenv("SPR_ITERATE_LANES", lanes)
         be = hip.Backend(amp, capacity=5)
+        be.set_variants(0, row_variant)
         be.set_population(rho, supp)
         be.iterate("RAAR", 7, 0.87, 0.0)
         be.iterate("HIO", 3, 0.9, 0.0)
@@ -110,15 +113,18 @@ def test_sampled_timing_counts(hip):
 
 
 
+@pytest.mark.parametrize("n,row_variant", [(128, 0), (256, 3)])
 @pytest.mark.parametrize("alg,beta", [("HIO", 0.9), ("RAAR", 0.87), ("ER", 0.0)])
-def test_chunked_blocks_are_bit_identical(hip, alg, beta):
+def test_chunked_blocks_are_bit_identical(hip, alg, beta, n, row_variant):
     """Chunked iteration blocks (2, 3 and 1 individuals per chunk, the last chunk
-    shorter) compute exactly what one block over the whole population does."""
-    n, P = 128, 7
+    shorter) compute exactly what one block over the whole population does,
+    with the state stored (128^2) or recomputed (row variant 3 at 256^2)."""
+    P = 7
     amp, rho, supp = case(n, P, seed=8)
     out = []
     for per in (0, 2, 3, 1):
         be = hip.Backend(amp, capacity=P)
+        be.set_variants(0, row_variant)
         be.set_chunk(per * 17 * n * n)
         be.set_population(rho, supp)
         assert be.chunk()[1] == (P if per == 0 else -(-P // -(-P // per)))

@@ -114,8 +114,10 @@ def test_schedule_precision_1024(hip, orc):
     RAAR 25, SW, HIO 25, ER 25, by the FP32 GPU path, the FP32 oracle and the
     FP64 oracle from the same start. HIO and RAAR amplify rounding, so the bars
     are relative: the GPU's final densities and Fourier errors are as close to
-    the FP64 run as the FP32 oracle's are (within 4x), and the supports after
-    shrink-wrap differ from the FP64 run's in few pixels."""
+    the FP64 run as the FP32 oracle's are (within 4x), and so are the supports
+    after shrink-wrap (pixels differing from the FP64 run's: within 4x of the
+    FP32 oracle's count, 128 of 161 033 on this case, and under 1 % of the
+    support)."""
     n, P = 1024, 2
     amp, rho, supp = octa_case(n, P, seed=35)
     seq = [("RAAR", [25, 0.87]), ("SW", [3.0, 0.03]), ("RAAR", [25, 0.87]), ("SW", [2.0, 0.03]),
@@ -142,9 +144,11 @@ def test_schedule_precision_1024(hip, orc):
     f64, fg, f32 = ef(r64, s64), ef(got, gs), ef(r32, s32)
     d_gpu, d_32 = np.abs(fg - f64).max(), np.abs(f32 - f64).max()
     flips = int((gs != (s64 != 0)).sum())
+    flips32 = int(((s32 != 0) != (s64 != 0)).sum())
     print(f"1024^2 schedule: densities rel-L2 vs FP64: GPU {e_gpu:.2e}, FP32 oracle {e_32:.2e}; "
           f"E_F FP64 {np.round(f64, 5).tolist()} GPU {np.round(fg, 5).tolist()} FP32 {np.round(f32, 5).tolist()}; "
-          f"support pixels differing from FP64: {flips} of {int((s64 != 0).sum())}")
+          f"support pixels differing from FP64: GPU {flips}, FP32 oracle {flips32} of {int((s64 != 0).sum())}")
     assert e_gpu < 4 * max(e_32, 1e-6)
     assert d_gpu <= 4 * max(d_32, 1e-4 * f64.max())
-    assert flips <= 1e-3 * (s64 != 0).sum() + 8
+    assert flips <= 4 * max(flips32, 8)
+    assert flips <= 1e-2 * (s64 != 0).sum()

@@ -28,6 +28,12 @@ SAME_PLAN = [(256, "col", 1, 0), (256, "col", 2, 0), (384, "col", 1, 0), (384, "
              (1024, "row", 4, 1), (2048, "row", 2, 1), (512, "col", 7, 0), (768, "col", 7, 0)]
 
 
+# (size, row variant that recomputes rho_n, its twin that stores it: same plan,
+# threads and padding)
+RECOMPUTED = [(256, 3, 2), (384, 3, 2), (512, 3, 0), (512, 4, 2), (768, 4, 3), (1024, 6, 5), (1024, 7, 1),
+              (2048, 4, 3), (2048, 5, 2)]
+
+
 def pairs(n):
     nc, nr = hipcore.variant_counts(n)
     return [(c, 0) for c in range(nc)] + [(0, r) for r in range(1, nr)]
@@ -65,6 +71,39 @@ def test_same_plan_variants_agree(hip, n, kind, v, ref):
         be.close()
     e = rel_l2(out[0], out[1])
     print(f"n={n} {kind} variant {v} vs {ref}: rel-L2 {e:.1e}")
+    assert e < 2e-6
+
+
+def test_row_variant_info():
+    for n, rs, stored in RECOMPUTED:
+        assert hipcore.row_variant_recomputes_state(n, rs)
+        assert not hipcore.row_variant_recomputes_state(n, stored)
+    with pytest.raises(hipcore.SpringHipError):
+        hipcore.row_variant_recomputes_state(512, 99)
+    with pytest.raises(hipcore.SpringHipError):
+        hipcore.row_variant_recomputes_state(500, 0)
+
+
+@pytest.mark.parametrize("n,rs,stored", RECOMPUTED)
+@pytest.mark.parametrize("alg,beta", [("HIO", 0.9), ("RAAR", 0.87)])
+def test_recomputed_state_agrees_with_the_stored_state(hip, n, rs, stored, alg, beta):
+    """A row variant that recomputes rho_n from its row transform (rowIFFT of
+    the rowFFT the iteration keeps, / N) instead of reading it runs one more
+    transform; one iteration agrees with its stored-state twin within relative
+    L2 2e-6 (float rounding), with the supports unchanged."""
+    P = 2 if n <= 1024 else 1
+    amp, rho, supp = case(n, P, seed=34)
+    out = []
+    for rv in (rs, stored):
+        be = hip.Backend(amp, capacity=P)
+        be.set_variants(0, rv)
+        be.set_population(rho, supp)
+        be.iterate(alg, 1, beta, 0.0)
+        out.append(be.get_population())
+        be.close()
+    assert np.array_equal(out[0][1], out[1][1])
+    e = rel_l2(out[0][0], out[1][0])
+    print(f"n={n} {alg}: row variant {rs} (recomputed state) vs {stored}: rel-L2 {e:.1e}")
     assert e < 2e-6
 
 
