@@ -4,7 +4,9 @@ Every variant must meet the single-step bar against the FP64 oracle (relative
 L2 <= 1e-5, supports unchanged). Variants that differ only in the tile width,
 the row-pass threads or the prefetch order run the same operations; the
 compiler may contract multiply-adds differently in each, so one step of two
-such variants must agree within relative L2 2e-6 (a few float roundings). The
+such variants must agree within relative L2 2e-6 (a few float roundings). A row
+variant that recomputes the HIO/RAAR state from its row transform instead of
+storing it must agree with its stored-state twin to the same bar. The
 autotuner must leave the resident population unchanged and keep the fastest
 pair it measured.
 """
