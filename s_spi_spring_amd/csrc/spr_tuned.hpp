@@ -12,8 +12,8 @@ constexpr TunedVariants kTuned[] = {
     {256, 1, 3},
     {384, 1, 2},
     {512, 6, 3},
-    {768, 2, 4},
-    {1024, 4, 1},
+    {768, 4, 4},
+    {1024, 4, 6},
     {2048, 2, 0},
 };
 
