@@ -20,12 +20,16 @@
 //   state recomputed (work holds rowFFT(rho_n); rho is read and written only at
 //   the ends of a block)
 //     column pass  proj <- colIFFT( P_M( colFFT(work) ) )                   16 B/px
-//     row pass     rho' = rowIFFT(proj)/N^2 ; rho_n = rowIFFT(work)/N ;
-//                  work <- rowFFT(update(rho', rho_n, S))             24.125 B/px
+//     row pass     HIO: rho' = rowIFFT(proj)/N^2 ; rho_n = rowIFFT(work)/N ;
+//                       work <- rowFFT(update(rho', rho_n, S))
+//                  RAAR: q = rowIFFT(2 proj/N^2 - work/N) (= 2 rho' - rho_n) ;
+//                       work <- proj/N - beta rowFFT(q on the rejected pixels)
+//                                                                     24.125 B/px
 // so HIO/RAAR move 48.125 or 40.125 B/px per individual-iteration (the second
-// form spends one more row transform to save the state's round trip); ER, which
-// needs no rho_n, moves 32.125 B/px in both. proj is the work2 buffer. (The
-// support packing reads 1 and writes 0.125 B/px once per block.)
+// form saves the state's round trip for one more row transform in HIO and none
+// in RAAR); ER, which needs no rho_n, moves 32.125 B/px in both. proj is the
+// work2 buffer. (The support packing reads 1 and writes 0.125 B/px once per
+// block.)
 #pragma once
 
 #include <cstdint>
@@ -476,6 +480,33 @@ __global__ __launch_bounds__((RowVariant<N, RV>::G::ROW_THREADS)) void k_row(Row
             v[r] = ld_stream(fin + off + r * T);
             if constexpr (PREFETCH >= 2) sn[r] = support_bit(sbits, off + r * T);
             if constexpr (MODE != ROW_ER && PREFETCH >= 1) bn[r] = ld_stream((RS ? work : rho) + off + r * T);
+        }
+        if constexpr (RS && MODE == ROW_RAAR) {
+            // RAAR needs rho' and rho_n only through q = 2 rho' - rho_n: its
+            // rejected branch beta rho_n + (1 - 2 beta) rho' is rho' - beta q. The
+            // transforms are linear, so q is one inverse transform of
+            // 2 proj / N^2 - work / N, and the new row transform is
+            // rowFFT(rho') - beta rowFFT(c) = proj / N - beta rowFFT(c), with c = q
+            // on the rejected pixels and 0 elsewhere: two row transforms, as with
+            // the state stored.
+            const float two_nn = 2.f * a.inv_nn;
+#pragma unroll
+            for (int r = 0; r < V; ++r) {
+                const float2 pr = v[r], wr = bn[r];
+                v[r] = make_float2(two_nn * pr.x - a.inv_n * wr.x, two_nn * pr.y - a.inv_n * wr.y);
+                bn[r] = make_float2(a.inv_n * pr.x, a.inv_n * pr.y);
+            }
+            fft_line_p<PL, -1, WV>(v, acc, tw, t);   // q
+#pragma unroll
+            for (int q = 0; q < V; ++q) {
+                const uint8_t s = PREFETCH >= 2 ? sn[q] : support_bit(sbits, off + q * T);
+                if (s != 0 && !phase_fails(v[q].x, v[q].y, a.pc)) v[q] = make_float2(0.f, 0.f);
+            }
+            fft_line_p<PL, +1, WV>(v, acc, tw, t);   // rowFFT(c)
+#pragma unroll
+            for (int q = 0; q < V; ++q)
+                st_stream(work + off + q * T, make_float2(bn[q].x - a.beta * v[q].x, bn[q].y - a.beta * v[q].y));
+            return;
         }
         if constexpr (RS && MODE != ROW_ER) {
             fft_line_p<PL, -1, WV>(bn, acc, tw, t);

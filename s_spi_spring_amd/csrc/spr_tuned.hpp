@@ -10,11 +10,11 @@ struct TunedVariants {
 };
 constexpr TunedVariants kTuned[] = {
     {256, 1, 3},
-    {384, 1, 2},
+    {384, 1, 3},
     {512, 6, 3},
     {768, 4, 4},
     {1024, 4, 6},
-    {2048, 2, 0},
+    {2048, 2, 5},
 };
 
 }  // namespace spr
