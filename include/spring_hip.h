@@ -308,9 +308,9 @@ int spr_stream(spr_ctx* ctx, void** stream);
  * accuracy bar against the oracle. Some row variants recompute the HIO/RAAR
  * state rho_n in each iteration from its row transform, which the iteration
  * keeps anyway, instead of reading and writing it (8 fewer bytes per pixel
- * and iteration for one more row transform); the column pass then writes its
- * projection to a second field buffer. The recomputed state agrees with the
- * stored one to float rounding.
+ * and iteration, for one more row transform in HIO and none in RAAR); the
+ * column pass then writes its projection to a second field buffer. The
+ * recomputed state agrees with the stored one to float rounding.
  *
  * spr_variants: number of column and row variants of size n.
  * spr_row_variant_info: whether row variant `variant` of size n recomputes
