@@ -620,16 +620,18 @@ int pack_support(spr_ctx* c, const uint8_t* supp, uint32_t* bits, int cnt, hipSt
 int queue_iterate_lanes(spr_ctx* c, int mode, int niter, const RowArgs& a0) {
     const int h = (c->P + 1) / 2, h2 = c->P - h;
     const size_t off = (size_t)h * c->n * c->n;
+    // With a row variant that recomputes rho_n the column passes project into work2.
+    const bool rs = row_recomputes(c->n, c->rowvar);
+    float2* pa = rs ? c->work2 : nullptr;
+    float2* pb = rs ? c->work2 + off : nullptr;
     RowArgs a = a0;
-    if (row_recomputes(c->n, c->rowvar)) a.proj = c->work2;   // the column passes project into work2
+    a.proj = pa;
     RowArgs b = a;
     b.rho += off;
     b.work += off;
     b.supp += off;
     b.sbits += off / 32;
-    if (b.proj) b.proj += off;
-    float2* pa = const_cast<float2*>(a.proj);
-    float2* pb = const_cast<float2*>(b.proj);
+    b.proj = pb;
     hipStream_t s1 = c->stream, s2 = c->lane2;
     SPR_TRY(pack_support(c, c->supp, c->sbits, c->P, s1));
     SPR_TRY(row_n(c, ROW_FWD, a, h, s1));
